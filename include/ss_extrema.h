@@ -1,0 +1,170 @@
+/*
+ * ss_extrema.h -- C ABI of the MI355X (gfx950) exact dual regional-extrema
+ * operator: an 8-bit grayscale image in, its exact 8-connected regional-maximum
+ * and regional-minimum masks (uint8 0/255) out.
+ *
+ * Semantics (identical to the reference's Sparse-Seam and Global-UF routes):
+ *   - flat zones are 8-connected sets of equal-valued pixels;
+ *   - a zone is a regional maximum iff no pixel of it has a strictly greater
+ *     8-neighbour, a regional minimum iff none has a strictly smaller one;
+ *   - neighbours outside the image are absent (never compared);
+ *   - masks are 255 on extremal zones and 0 elsewhere.
+ *
+ * The reference has no ABI: its operator is a C++ source-level API pulled into
+ * harness executables by #include (harness/primary/gpu_common_v0_6.cu:11-55).
+ * Each entry point below names the reference function it replaces.
+ *
+ * Conventions: every device pointer is caller-owned HIP device memory; no entry
+ * point allocates device memory or synchronises unless its comment says so;
+ * no entry point throws.  Work is enqueued on the caller's stream and is
+ * capturable into a hipGraph.  Calls that share a workspace must be ordered
+ * on one stream; distinct workspaces may run concurrently on distinct streams.
+ */
+#ifndef SS_EXTREMA_H
+#define SS_EXTREMA_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ss_status {
+    SS_OK = 0,
+    SS_ERR_ARG = 1,        /* bad dimensions, pitch, config or null pointer  */
+    SS_ERR_WORKSPACE = 2,  /* workspace too small or not 256-byte aligned    */
+    SS_ERR_HIP = 3,        /* a HIP runtime call failed                      */
+    SS_ERR_INVARIANT = 4,  /* a device invariant failed (see ss_stats.invariant) */
+    SS_ERR_CAPACITY = 5,   /* batch / serving capacity exceeded or admission refused */
+    SS_ERR_NO_DEVICE = 6   /* no gfx950 device / kernels not loadable        */
+} ss_status;
+
+/* Tile side T in {16, 32, 64} and threads per tile in {64, 128, 256, 512}
+ * (threads <= T*T).  Replaces the reference's compile-time SS_TILE /
+ * SS_CLASSIFY_THREADS (snapshot_sparse_seam_cuda_v1_0.cu:34-46,
+ * generated_sparse_seam_kernel_v0_7.cu:782-785). */
+typedef struct ss_config {
+    int tile;
+    int threads;
+} ss_config;
+
+/* Per-run counters, read back by ss_read_stats (synchronous). */
+typedef struct ss_stats {
+    uint32_t invariant;      /* 0 ok; 1 = more than 4T-4 seam fragments in a tile,
+                                2 = bad fragment record (reference flags 1 and 2,
+                                snapshot_sparse_seam_cuda_v1_0.cu:404-406,432-434) */
+    uint32_t dirty_tiles;    /* tiles whose seam zones changed after the global merge */
+    uint64_t fragments;      /* seam fragments in the dense array (the reference's
+                                active_fragments, snapshot_sparse_seam_cuda_v1_0.cu:852-863) */
+    uint64_t dsu_retries;    /* failed CAS attempts in the seam union-find */
+    uint64_t tiles;          /* tiles processed */
+} ss_stats;
+
+/* Human-readable name of a status code (static storage). */
+const char* ss_status_str(ss_status status);
+
+/* Bytes of device workspace needed for one w x h image at cfg (NULL = the
+ * default / autotuned config).  Replaces allocate_buffers
+ * (snapshot_sparse_seam_cuda_v1_0.cu:710-737): the reference allocates image,
+ * masks, a per-pixel record and 16 B per tile slot; here the caller owns image
+ * and masks and the workspace holds only per-tile and per-fragment state. */
+ss_status ss_workspace_size(int width, int height, const ss_config* cfg, size_t* bytes);
+
+/* Exact dual regional extrema of one image.  Replaces run_sparse_seam /
+ * run_sparse_resident (snapshot_sparse_seam_cuda_v1_0.cu:753-878,
+ * gpu_common_v0_6.cu:137-206) with the image already resident in HBM.
+ *   d_img    : width x height uint8 pixels, row r at d_img + r*pitch (pitch >= width)
+ *   d_max/min: outputs, row r at d + r*out_pitch (out_pitch >= width); both masks
+ *              are fully written (every pixel 0 or 255)
+ *   d_ws     : workspace of ws_bytes >= ss_workspace_size(...), 256-byte aligned
+ *   cfg      : NULL = default / autotuned
+ * Asynchronous on `stream`; invariant failures are reported by ss_read_stats. */
+ss_status ss_extrema_u8(const uint8_t* d_img, size_t pitch, int width, int height,
+                        uint8_t* d_max, uint8_t* d_min, size_t out_pitch,
+                        void* d_ws, size_t ws_bytes, const ss_config* cfg,
+                        hipStream_t stream);
+
+/* Synchronises `stream` and reads the counters of the last run that used d_ws.
+ * Returns SS_ERR_INVARIANT when the device invariant word is non-zero (the
+ * reference throws "Sparse-seam device invariant failed",
+ * snapshot_sparse_seam_cuda_v1_0.cu:840-850). */
+ss_status ss_read_stats(const void* d_ws, size_t ws_bytes, ss_stats* out, hipStream_t stream);
+
+/* Per-tile seam-fragment counts of the last run (tiles_y x tiles_x uint32),
+ * copied to host memory `out` (synchronous).  The reference reads the same
+ * counts back to sum active_fragments (snapshot_sparse_seam_cuda_v1_0.cu:852-863). */
+ss_status ss_read_tile_counts(const void* d_ws, size_t ws_bytes, uint32_t* out, size_t count,
+                              hipStream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched form: many images of uneven sizes in one pipeline.               */
+/* No reference counterpart: the reference loops over images, one          */
+/* allocate / run / free per image (gpu_common_v0_6.cu:584-745).            */
+/* ------------------------------------------------------------------------ */
+
+typedef struct ss_image_desc {
+    const uint8_t* img;   /* device pointer */
+    uint8_t* max;         /* device pointer */
+    uint8_t* min;         /* device pointer */
+    uint64_t pitch;       /* input row pitch in bytes  (>= width) */
+    uint64_t out_pitch;   /* output row pitch in bytes (>= width) */
+    int32_t width;
+    int32_t height;
+} ss_image_desc;
+
+/* Workspace for a batch: n images described on the HOST (only their sizes are
+ * read). */
+ss_status ss_batch_workspace_size(const ss_image_desc* images, int n, const ss_config* cfg,
+                                  size_t* bytes);
+
+/* Runs the whole batch: one persistent tile-stage launch fed by a device-side
+ * tile work queue, then the seam stages over all images at once.  `images` is
+ * a HOST array; its descriptors are staged into the workspace with one
+ * asynchronous copy.  Not capturable (the copy reads pageable memory); use
+ * ss_batch_plan_* for graph replay. */
+ss_status ss_extrema_batch_u8(const ss_image_desc* images, int n, void* d_ws, size_t ws_bytes,
+                              const ss_config* cfg, hipStream_t stream);
+
+/* Graph-replayable batch plan: descriptors are staged once, the pipeline is
+ * captured into a hipGraph, and each ss_batch_plan_run replays it. */
+typedef struct ss_batch_plan ss_batch_plan;
+ss_status ss_batch_plan_create(const ss_image_desc* images, int n, void* d_ws, size_t ws_bytes,
+                               const ss_config* cfg, hipStream_t stream, ss_batch_plan** plan);
+ss_status ss_batch_plan_run(ss_batch_plan* plan, hipStream_t stream);
+ss_status ss_batch_plan_destroy(ss_batch_plan* plan);
+
+/* ------------------------------------------------------------------------ */
+/* Autotuner: picks (tile, threads) by timing the candidates on the device. */
+/* Replaces the reference's offline 9-build sweep and its selection metric, */
+/* sum(pixels) / sum(per-image median) (run_config_study_v0_8.ps1:85-109).  */
+/* ------------------------------------------------------------------------ */
+
+/* Times every candidate (tile, threads) on the given resident images
+ * (reps per candidate, median per image), writes the best to *best and makes it
+ * the process-wide default used when cfg == NULL.  Allocates a scratch
+ * workspace with hipMalloc and synchronises.  `throughput` (optional) receives
+ * the winner's pixels/s. */
+ss_status ss_autotune(const ss_image_desc* images, int n, int reps, hipStream_t stream,
+                      ss_config* best, double* throughput);
+
+/* Sets / reads the process-wide default config (used when cfg == NULL). */
+ss_status ss_set_default_config(const ss_config* cfg);
+ss_status ss_get_default_config(ss_config* cfg);
+
+/* ------------------------------------------------------------------------ */
+/* Measurement aid (no reference counterpart; the reference times the whole */
+/* kernel sequence with one cudaEvent pair, gpu_common_v0_6.cu:338-347).    */
+/* ------------------------------------------------------------------------ */
+
+/* Records tile_begin / tile_end around the tile-stage kernel of every pipeline
+ * enqueued afterwards by the calling thread, on that pipeline's stream, so the
+ * dominant kernel can be timed in place.  Pass two nulls to turn it off. */
+ss_status ss_profile_events(hipEvent_t tile_begin, hipEvent_t tile_end);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SS_EXTREMA_H */

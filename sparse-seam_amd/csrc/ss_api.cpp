@@ -1,0 +1,275 @@
+// ss_api.cpp -- C ABI of the single-image and batched operator (see
+// include/ss_extrema.h for the contract and the reference functions replaced).
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "ss_internal.hpp"
+
+namespace ssx {
+
+static std::mutex g_cfg_mu;
+static ss_config g_default_cfg = {32, 256};
+
+ss_config resolve_config(const ss_config* cfg) {
+    if (cfg) return *cfg;
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    return g_default_cfg;
+}
+
+int device_cus() {
+    static std::mutex mu;
+    static std::vector<int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
+    if (!cache[dev]) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        cache[dev] = cus;
+    }
+    return cache[dev];
+}
+
+ProfileHook& profile_hook() {
+    static thread_local ProfileHook hook;
+    return hook;
+}
+
+static bool aligned256(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 255u) == 0; }
+
+}  // namespace ssx
+
+using namespace ssx;
+
+struct ss_batch_plan {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+const char* ss_status_str(ss_status s) {
+    switch (s) {
+    case SS_OK: return "ok";
+    case SS_ERR_ARG: return "invalid argument";
+    case SS_ERR_WORKSPACE: return "workspace too small or misaligned";
+    case SS_ERR_HIP: return "HIP runtime error";
+    case SS_ERR_INVARIANT: return "device invariant failed";
+    case SS_ERR_CAPACITY: return "capacity exceeded";
+    case SS_ERR_NO_DEVICE: return "no usable device";
+    }
+    return "unknown status";
+}
+
+ss_status ss_profile_events(hipEvent_t tile_begin, hipEvent_t tile_end) {
+    if ((tile_begin == nullptr) != (tile_end == nullptr)) return SS_ERR_ARG;
+    profile_hook().tile_begin = tile_begin;
+    profile_hook().tile_end = tile_end;
+    return SS_OK;
+}
+
+ss_status ss_set_default_config(const ss_config* cfg) {
+    if (!cfg || !config_valid(cfg->tile, cfg->threads)) return SS_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    g_default_cfg = *cfg;
+    return SS_OK;
+}
+
+ss_status ss_get_default_config(ss_config* cfg) {
+    if (!cfg) return SS_ERR_ARG;
+    *cfg = resolve_config(nullptr);
+    return SS_OK;
+}
+
+ss_status ss_workspace_size(int width, int height, const ss_config* cfg, size_t* bytes) {
+    if (!bytes || width <= 0 || height <= 0) return SS_ERR_ARG;
+    const ss_config c = resolve_config(cfg);
+    if (!config_valid(c.tile, c.threads)) return SS_ERR_ARG;
+    const int64_t nt = tiles_of(width, height, c.tile);
+    if (nt * (4 * c.tile - 4) > (int64_t)0xFFFFFFF0u) return SS_ERR_ARG;
+    *bytes = make_layout(nt, c.tile, 1).bytes;
+    return SS_OK;
+}
+
+ss_status ss_extrema_u8(const uint8_t* d_img, size_t pitch, int width, int height, uint8_t* d_max,
+                        uint8_t* d_min, size_t out_pitch, void* d_ws, size_t ws_bytes,
+                        const ss_config* cfg, hipStream_t stream) {
+    if (!d_img || !d_max || !d_min || width <= 0 || height <= 0) return SS_ERR_ARG;
+    if (pitch < (size_t)width || out_pitch < (size_t)width) return SS_ERR_ARG;
+    const ss_config c = resolve_config(cfg);
+    if (!config_valid(c.tile, c.threads)) return SS_ERR_ARG;
+    const int64_t nt = tiles_of(width, height, c.tile);
+    if (nt * (4 * c.tile - 4) > (int64_t)0xFFFFFFF0u) return SS_ERR_ARG;
+    const Layout L = make_layout(nt, c.tile, 1);
+    if (!d_ws || ws_bytes < L.bytes || !aligned256(d_ws)) return SS_ERR_WORKSPACE;
+    Images ims{};
+    ims.one = ImgDev{d_img, d_max, d_min, (int64_t)pitch, (int64_t)out_pitch, width, height,
+                     (width + c.tile - 1) / c.tile, (height + c.tile - 1) / c.tile, 0};
+    ims.table = nullptr;
+    ims.count = 1;
+    const Ws ws = make_ws(d_ws, L, c.threads);
+    const hipError_t e = launch_pipeline(ims, ws, c.tile, c.threads, false, device_cus(), stream);
+    return e == hipSuccess ? SS_OK : SS_ERR_HIP;
+}
+
+ss_status ss_read_stats(const void* d_ws, size_t ws_bytes, ss_stats* out, hipStream_t stream) {
+    if (!d_ws || !out || ws_bytes < sizeof(WsHeader)) return SS_ERR_ARG;
+    WsHeader h;
+    if (hipMemcpyAsync(&h, d_ws, sizeof(h), hipMemcpyDeviceToHost, stream) != hipSuccess) return SS_ERR_HIP;
+    if (hipStreamSynchronize(stream) != hipSuccess) return SS_ERR_HIP;
+    if (h.magic != WS_MAGIC) return SS_ERR_WORKSPACE;
+    out->invariant = h.error;
+    out->dirty_tiles = h.dirty_count;
+    out->fragments = h.total;
+    out->dsu_retries = h.retries;
+    out->tiles = h.tiles;
+    return h.error ? SS_ERR_INVARIANT : SS_OK;
+}
+
+ss_status ss_read_tile_counts(const void* d_ws, size_t ws_bytes, uint32_t* out, size_t count,
+                              hipStream_t stream) {
+    if (!d_ws || !out || ws_bytes < sizeof(WsHeader) + count * 4) return SS_ERR_ARG;
+    const char* src = static_cast<const char*>(d_ws) + sizeof(WsHeader);
+    if (hipMemcpyAsync(out, src, count * 4, hipMemcpyDeviceToHost, stream) != hipSuccess) return SS_ERR_HIP;
+    if (hipStreamSynchronize(stream) != hipSuccess) return SS_ERR_HIP;
+    for (size_t i = 0; i < count; ++i) out[i] &= COUNT_MASK;
+    return SS_OK;
+}
+
+// ---- batch ----------------------------------------------------------------
+
+static ss_status batch_prepare(const ss_image_desc* images, int n, const ss_config& c,
+                               std::vector<ImgDev>& table, Layout& L) {
+    if (!images || n <= 0) return SS_ERR_ARG;
+    if (!config_valid(c.tile, c.threads)) return SS_ERR_ARG;
+    table.resize((size_t)n);
+    int64_t tiles = 0;
+    for (int i = 0; i < n; ++i) {
+        const ss_image_desc& d = images[i];
+        if (d.width <= 0 || d.height <= 0 || d.pitch < (uint64_t)d.width || d.out_pitch < (uint64_t)d.width)
+            return SS_ERR_ARG;
+        ImgDev& t = table[(size_t)i];
+        t.img = d.img;
+        t.mx = d.max;
+        t.mn = d.min;
+        t.pitch = (int64_t)d.pitch;
+        t.opitch = (int64_t)d.out_pitch;
+        t.w = d.width;
+        t.h = d.height;
+        t.tiles_x = (d.width + c.tile - 1) / c.tile;
+        t.tiles_y = (d.height + c.tile - 1) / c.tile;
+        t.tile_begin = tiles;
+        tiles += (int64_t)t.tiles_x * t.tiles_y;
+    }
+    if (tiles * (4 * c.tile - 4) > (int64_t)0xFFFFFFF0u) return SS_ERR_CAPACITY;
+    L = make_layout(tiles, c.tile, n);
+    return SS_OK;
+}
+
+ss_status ss_batch_workspace_size(const ss_image_desc* images, int n, const ss_config* cfg,
+                                  size_t* bytes) {
+    if (!bytes) return SS_ERR_ARG;
+    std::vector<ImgDev> table;
+    Layout L;
+    const ss_status st = batch_prepare(images, n, resolve_config(cfg), table, L);
+    if (st != SS_OK) return st;
+    *bytes = L.bytes;
+    return SS_OK;
+}
+
+static ss_status batch_enqueue(const ss_image_desc* images, int n, void* d_ws, size_t ws_bytes,
+                               const ss_config& c, hipStream_t stream, bool stage_descs) {
+    std::vector<ImgDev> table;
+    Layout L;
+    ss_status st = batch_prepare(images, n, c, table, L);
+    if (st != SS_OK) return st;
+    for (int i = 0; i < n; ++i)
+        if (!images[i].img || !images[i].max || !images[i].min) return SS_ERR_ARG;
+    if (!d_ws || ws_bytes < L.bytes || !aligned256(d_ws)) return SS_ERR_WORKSPACE;
+    Images ims{};
+    ims.count = n;
+    if (n == 1) {
+        ims.one = table[0];
+        ims.table = nullptr;
+    } else {
+        ImgDev* dst = reinterpret_cast<ImgDev*>(static_cast<char*>(d_ws) + L.off_descs);
+        if (stage_descs) {
+            if (hipMemcpyAsync(dst, table.data(), table.size() * sizeof(ImgDev), hipMemcpyHostToDevice,
+                               stream) != hipSuccess)
+                return SS_ERR_HIP;
+        }
+        ims.table = dst;
+    }
+    const Ws ws = make_ws(d_ws, L, c.threads);
+    const hipError_t e = launch_pipeline(ims, ws, c.tile, c.threads, true, device_cus(), stream);
+    return e == hipSuccess ? SS_OK : SS_ERR_HIP;
+}
+
+ss_status ss_extrema_batch_u8(const ss_image_desc* images, int n, void* d_ws, size_t ws_bytes,
+                              const ss_config* cfg, hipStream_t stream) {
+    return batch_enqueue(images, n, d_ws, ws_bytes, resolve_config(cfg), stream, true);
+}
+
+ss_status ss_batch_plan_create(const ss_image_desc* images, int n, void* d_ws, size_t ws_bytes,
+                               const ss_config* cfg, hipStream_t stream, ss_batch_plan** plan) {
+    if (!plan) return SS_ERR_ARG;
+    *plan = nullptr;
+    const ss_config c = resolve_config(cfg);
+    std::vector<ImgDev> table;
+    Layout L;
+    ss_status st = batch_prepare(images, n, c, table, L);
+    if (st != SS_OK) return st;
+    if (!d_ws || ws_bytes < L.bytes || !aligned256(d_ws)) return SS_ERR_WORKSPACE;
+    // Stage the descriptors once, outside the graph.
+    if (n > 1) {
+        ImgDev* dst = reinterpret_cast<ImgDev*>(static_cast<char*>(d_ws) + L.off_descs);
+        if (hipMemcpyAsync(dst, table.data(), table.size() * sizeof(ImgDev), hipMemcpyHostToDevice,
+                           stream) != hipSuccess)
+            return SS_ERR_HIP;
+        if (hipStreamSynchronize(stream) != hipSuccess) return SS_ERR_HIP;
+    }
+    ss_batch_plan* p = new (std::nothrow) ss_batch_plan();
+    if (!p) return SS_ERR_CAPACITY;
+    if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        delete p;
+        return SS_ERR_HIP;
+    }
+    st = batch_enqueue(images, n, d_ws, ws_bytes, c, stream, false);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(stream, &g);
+    if (st != SS_OK || ec != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        delete p;
+        return st != SS_OK ? st : SS_ERR_HIP;
+    }
+    p->graph = g;
+    if (hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        delete p;
+        return SS_ERR_HIP;
+    }
+    *plan = p;
+    return SS_OK;
+}
+
+ss_status ss_batch_plan_run(ss_batch_plan* plan, hipStream_t stream) {
+    if (!plan || !plan->exec) return SS_ERR_ARG;
+    return hipGraphLaunch(plan->exec, stream) == hipSuccess ? SS_OK : SS_ERR_HIP;
+}
+
+ss_status ss_batch_plan_destroy(ss_batch_plan* plan) {
+    if (!plan) return SS_OK;
+    if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+    if (plan->graph) (void)hipGraphDestroy(plan->graph);
+    delete plan;
+    return SS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,222 @@
+// ss_device.hpp -- device-side building blocks of the dual regional-extrema
+// pipeline for gfx950 (wave64, 160 KB LDS per CU, 8 XCDs).
+//
+// Pipeline for one launch set (single image or a whole batch):
+//   1. tile stage   one workgroup per T x T tile: stage the tile plus a
+//                   one-pixel halo in LDS, label tile-local flat zones
+//                   (Komura init + min-root union-find in LDS), OR the
+//                   greater / smaller / touches-seam evidence into each zone
+//                   root, write both masks for every pixel (final for
+//                   tile-contained zones, provisional for seam zones), number
+//                   the seam zones ("fragments") and publish per perimeter
+//                   pixel its fragment and its cross-seam equality bits.
+//   2. compaction   exclusive scan of the per-tile fragment counts gives every
+//                   fragment a dense global id; the dense union-find arrays
+//                   are initialised from the per-tile slots.
+//   3. seam merge   lock-free min-root CAS union with path halving over the
+//                   dense fragment ids, one thread per seam position.
+//   4. resolve      full compression and OR of the local evidence into roots.
+//   5. mark         fragments whose resolved evidence differs from the local
+//                   evidence are flagged; their tiles go to a dirty-tile list.
+//   6. write-back   dirty tiles are relabelled in LDS (the labels are the same
+//                   deterministic min-index roots as in step 1) and only the
+//                   pixels of changed fragments are rewritten.
+//
+// Everything is exact integer work; no MFMA (nothing here is a contraction).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ssx {
+
+constexpr uint32_t F_G = 1u;      // a strictly greater 8-neighbour exists
+constexpr uint32_t F_S = 2u;      // a strictly smaller 8-neighbour exists
+constexpr uint32_t F_SEAM = 4u;   // an equal 8-neighbour lies in another tile
+constexpr uint32_t LMASK = 0xFFFu;      // label bits (T*T <= 4096)
+constexpr int FLAG_SHIFT = 16;          // zone evidence OR'ed at the root
+constexpr int FRAG_SHIFT = 20;          // fragment id + 1 stored at seam roots
+constexpr uint32_t DB_CHANGED = 16u;    // dense bits: resolved output differs
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_CHUNK = SCAN_BLOCK * SCAN_ITEMS;  // tiles per scan block
+constexpr int NUM_XCD = 8;
+
+// One image of a launch set.  tile_begin is the global id of its first tile.
+struct ImgDev {
+    const uint8_t* img;
+    uint8_t* mx;
+    uint8_t* mn;
+    int64_t pitch;
+    int64_t opitch;
+    int32_t w, h, tiles_x, tiles_y;
+    int64_t tile_begin;
+};
+
+struct Images {
+    ImgDev one;           // count == 1: the image itself (table may be null)
+    const ImgDev* table;  // count > 1: device table sorted by tile_begin
+    int count;
+};
+
+// Workspace header (first 256 bytes of every workspace).
+struct WsHeader {
+    uint32_t magic;
+    uint32_t error;        // invariant word of the last run
+    uint32_t reserved;
+    uint32_t dirty_count;
+    uint32_t total;        // fragments in the dense array
+    int32_t tile;
+    int32_t threads;
+    uint32_t nblocks;      // scan blocks
+    uint64_t retries;      // failed CAS in the seam union-find
+    uint64_t tiles;
+    uint32_t queue[NUM_XCD];   // per-XCD work-queue tickets (persistent tile stage)
+    uint32_t queue_end[NUM_XCD];
+    uint32_t pad[64 - 10 - 2 * NUM_XCD - 2];
+};
+static_assert(sizeof(WsHeader) == 256, "header must be 256 bytes");
+
+// counts[t] holds the tile's seam-fragment count in bits 0-27 and the
+// tile-stage invariant code in bits 28-31 (folded into WsHeader::error by the
+// scan, so no per-run clearing is needed before the tile stage).
+constexpr uint32_t COUNT_MASK = 0x0FFFFFFFu;
+constexpr int COUNT_ERR_SHIFT = 28;
+
+struct Ws {
+    WsHeader* hdr;
+    uint32_t* counts;    // [tiles] seam fragments per tile (+ invariant code)
+    uint32_t* base;      // [tiles] dense id of the tile's first fragment
+    uint32_t* bsum;      // [nblocks + 1] scan block sums (last = total)
+    uint32_t* berr;      // [nblocks] OR of the tile invariant codes per scan block
+    uint8_t* sbits;      // [tiles * S] local G/S evidence per fragment slot
+    uint16_t* sroot;     // [tiles * S] tile-local root index per fragment slot
+    uint8_t* rec;        // [tiles * 4T] fragment id of seam perimeter pixels
+                         //   [0,T) top row, [T,2T) bottom row, [2T,3T) left col, [3T,4T) right col
+    uint8_t* eq;         // [tiles * 2T] cross-seam equality bits: [0,T) right col, [T,2T) bottom row
+    uint32_t* dparent;   // [tiles * S] dense union-find parents
+    uint32_t* dbits;     // [tiles * S] bits 0-1 local, 2-3 resolved, 4 changed
+    uint32_t* dirty;     // [tiles] dirty tile list
+    int64_t ntiles;
+    uint32_t nblocks;    // scan blocks = ceil(ntiles / SCAN_CHUNK)
+    int tile;            // T
+    int threads;         // threads per tile
+    int slots;           // S = 4T - 4
+};
+
+constexpr uint32_t WS_MAGIC = 0x53535831u;  // "SSX1"
+
+// Image of a global tile id.  MULTI is a template parameter so that the
+// single-image kernels read the descriptor straight from the kernel
+// arguments and never form a pointer that could alias either source.
+template <bool MULTI>
+__device__ __forceinline__ ImgDev locate(const Images& ims, int64_t tile) {
+    if constexpr (!MULTI) {
+        return ims.one;
+    } else {
+        int lo = 0, hi = ims.count - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (ims.table[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
+        }
+        return ims.table[lo];
+    }
+}
+
+// XCD-aware logical block id: workgroups are dispatched round-robin over the
+// 8 XCDs, so block b runs on XCD b % 8.  Give each XCD a contiguous run of
+// logical ids so that horizontally adjacent tiles (which share image and mask
+// cache lines) are processed by the same L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t g) {
+    const int64_t per = g / NUM_XCD, rem = g % NUM_XCD;
+    const int64_t k = b % NUM_XCD, i = b / NUM_XCD;
+    return (k < rem ? k * (per + 1) : rem * (per + 1) + (k - rem) * per) + i;
+}
+
+// ---- LDS union-find (workgroup scope) ------------------------------------
+
+__device__ __forceinline__ uint32_t lds_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ uint32_t lfind(const uint32_t* lab, uint32_t x) {
+    uint32_t r = lds_load(lab + x);
+    for (;;) {
+        const uint32_t n = lds_load(lab + r);
+        if (n == r) return r;
+        r = n;
+    }
+}
+
+__device__ __forceinline__ void lunion(uint32_t* lab, uint32_t a, uint32_t b) {
+    for (;;) {
+        a = lfind(lab, a);
+        b = lfind(lab, b);
+        if (a == b) return;
+        const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+        uint32_t expected = hi;
+        if (__hip_atomic_compare_exchange_strong(lab + hi, &expected, lo, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+            return;
+        a = hi;
+        b = lo;
+    }
+}
+
+// ---- global union-find over dense fragment ids (agent scope) -------------
+
+__device__ __forceinline__ uint32_t g_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void g_store(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Find with path halving.  Only non-roots are rewritten, and always to an
+// ancestor, so the stores never race with a link (links CAS roots only).
+__device__ __forceinline__ uint32_t gfind(uint32_t* parent, uint32_t x) {
+    for (;;) {
+        const uint32_t p = g_load(parent + x);
+        if (p == x) return x;
+        const uint32_t gp = g_load(parent + p);
+        if (gp == p) return p;
+        g_store(parent + x, gp);
+        x = gp;
+    }
+}
+
+// Min-root union: the larger root is linked under the smaller one by CAS.
+__device__ __forceinline__ void gunion(uint32_t* parent, uint32_t a, uint32_t b, uint32_t& retries) {
+    for (;;) {
+        a = gfind(parent, a);
+        b = gfind(parent, b);
+        if (a == b) return;
+        const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+        uint32_t expected = hi;
+        if (__hip_atomic_compare_exchange_strong(parent + hi, &expected, lo, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return;
+        ++retries;
+        a = hi;
+        b = lo;
+    }
+}
+
+// ---- wave / block scans ---------------------------------------------------
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t n = __shfl_up(v, d, 64);
+        if (lane >= d) v += n;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+}  // namespace ssx

@@ -1,0 +1,108 @@
+// ss_internal.hpp -- host-side declarations shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cstddef>
+
+#include "ss_device.hpp"
+#include "ss_extrema.h"
+
+namespace ssx {
+
+// Byte offsets of the workspace regions (all 256-byte aligned).
+struct Layout {
+    int64_t ntiles = 0;
+    int tile = 0;
+    int slots = 0;
+    uint32_t nblocks = 0;
+    size_t off_counts = 0, off_base = 0, off_bsum = 0, off_berr = 0, off_sbits = 0,
+           off_sroot = 0, off_rec = 0, off_eq = 0, off_dparent = 0, off_dbits = 0, off_dirty = 0,
+           off_descs = 0;
+    size_t bytes = 0;
+};
+
+inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+inline Layout make_layout(int64_t ntiles, int T, int ndesc) {
+    Layout L;
+    L.ntiles = ntiles;
+    L.tile = T;
+    L.slots = 4 * T - 4;
+    L.nblocks = (uint32_t)((ntiles + SCAN_CHUNK - 1) / SCAN_CHUNK);
+    const size_t nt = (size_t)ntiles, S = (size_t)L.slots;
+    size_t o = sizeof(WsHeader);
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = align256(o + bytes);
+        return at;
+    };
+    L.off_counts = take(nt * 4);
+    L.off_base = take(nt * 4);
+    L.off_bsum = take(((size_t)L.nblocks + 1) * 4);
+    L.off_berr = take((size_t)L.nblocks * 4 + 4);
+    L.off_sbits = take(nt * S);
+    L.off_sroot = take(nt * S * 2);
+    L.off_rec = take(nt * 4 * (size_t)T);
+    L.off_eq = take(nt * 2 * (size_t)T);
+    L.off_dparent = take(nt * S * 4);
+    L.off_dbits = take(nt * S * 4);
+    L.off_dirty = take(nt * 4);
+    L.off_descs = take((size_t)(ndesc > 1 ? ndesc : 0) * sizeof(ImgDev));
+    L.bytes = o;
+    return L;
+}
+
+inline Ws make_ws(void* base, const Layout& L, int threads) {
+    char* b = static_cast<char*>(base);
+    Ws w;
+    w.hdr = reinterpret_cast<WsHeader*>(b);
+    w.counts = reinterpret_cast<uint32_t*>(b + L.off_counts);
+    w.base = reinterpret_cast<uint32_t*>(b + L.off_base);
+    w.bsum = reinterpret_cast<uint32_t*>(b + L.off_bsum);
+    w.berr = reinterpret_cast<uint32_t*>(b + L.off_berr);
+    w.sbits = reinterpret_cast<uint8_t*>(b + L.off_sbits);
+    w.sroot = reinterpret_cast<uint16_t*>(b + L.off_sroot);
+    w.rec = reinterpret_cast<uint8_t*>(b + L.off_rec);
+    w.eq = reinterpret_cast<uint8_t*>(b + L.off_eq);
+    w.dparent = reinterpret_cast<uint32_t*>(b + L.off_dparent);
+    w.dbits = reinterpret_cast<uint32_t*>(b + L.off_dbits);
+    w.dirty = reinterpret_cast<uint32_t*>(b + L.off_dirty);
+    w.ntiles = L.ntiles;
+    w.nblocks = L.nblocks;
+    w.tile = L.tile;
+    w.threads = threads;
+    w.slots = L.slots;
+    return w;
+}
+
+inline bool config_valid(int T, int NT) {
+    if (T != 16 && T != 32 && T != 64) return false;
+    if (NT != 64 && NT != 128 && NT != 256 && NT != 512) return false;
+    return NT <= T * T;
+}
+
+inline int64_t tiles_of(int w, int h, int T) {
+    return (int64_t)((w + T - 1) / T) * (int64_t)((h + T - 1) / T);
+}
+
+// Resolved config: cfg, else the process default.
+ss_config resolve_config(const ss_config* cfg);
+
+// Compute units of the current device (cached per device).
+int device_cus();
+
+// Events recorded around the tile stage of every pipeline enqueued by the
+// calling thread (both null = off).  Set by ss_profile_events.
+struct ProfileHook {
+    hipEvent_t tile_begin = nullptr;
+    hipEvent_t tile_end = nullptr;
+};
+ProfileHook& profile_hook();
+
+// Enqueues the whole pipeline on `s`.
+hipError_t launch_pipeline(const Images& ims, const Ws& ws, int T, int NT, bool persistent,
+                           int cus, hipStream_t s);
+
+}  // namespace ssx

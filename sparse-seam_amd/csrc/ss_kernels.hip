@@ -1,0 +1,597 @@
+// ss_kernels.hip -- gfx950 kernels of the exact dual regional-extrema path and
+// the host-side sequencer that enqueues them.
+//
+// Reference behaviour followed (harness/primary/snapshot_sparse_seam_cuda_v1_0.cu):
+//   tile-local Komura labels        :194-289  -> tile_label()
+//   classify / number / emit        :291-475  -> tile_stage()
+//   seam merge (3 pairs per seam px):508-581  -> k_seam_merge
+//   compress + reduce               :583-619  -> k_resolve
+//   materialise seam records        :477-506  -> k_mark + k_writeback
+// The record-per-pixel write-back of the reference (1 B written and re-read for
+// every pixel) is replaced by provisional masks plus an LDS relabel of the
+// (usually few) tiles whose seam zones change.
+#include "ss_device.hpp"
+#include "ss_internal.hpp"
+
+namespace ssx {
+
+template <int T>
+struct TileSmem {
+    uint8_t img[(T + 2) * (T + 2)];   // tile + one-pixel halo (replicated at image borders)
+    uint8_t pflag[T * T];             // per-pixel G/S/SEAM evidence (write-back: root -> fragment map)
+    uint32_t lab[T * T];              // labels; at roots also zone evidence and fragment id
+    uint8_t eqr[T];
+    uint8_t eqb[T];
+    uint32_t count;
+    uint32_t err;
+    uint32_t next;                    // work-queue broadcast
+};
+
+// Loads the tile (and, with HALO, its one-pixel halo) into LDS.  Halo pixels
+// outside the image replicate the nearest image pixel: such a value always
+// belongs to the pixel itself or to one of its genuine 8-neighbours, so the
+// 3x3 greater / smaller tests equal the "absent neighbour" rule exactly.
+template <int T, int NT, bool HALO>
+__device__ __forceinline__ void tile_load(TileSmem<T>& sm, const ImgDev& im, int x0, int y0,
+                                          int tw, int th) {
+    constexpr int P = T + 2;
+    if (HALO) {
+        for (int i = threadIdx.x; i < P * P; i += NT) {
+            const int ly = i / P - 1, lx = i % P - 1;
+            if (ly > th || lx > tw) continue;
+            int gy = y0 + ly, gx = x0 + lx;
+            gy = gy < 0 ? 0 : (gy >= im.h ? im.h - 1 : gy);
+            gx = gx < 0 ? 0 : (gx >= im.w ? im.w - 1 : gx);
+            sm.img[i] = im.img[(int64_t)gy * im.pitch + gx];
+        }
+    } else {
+        for (int p = threadIdx.x; p < T * T; p += NT) {
+            const int lx = p % T, ly = p / T;
+            if (lx >= tw || ly >= th) continue;
+            sm.img[(ly + 1) * P + lx + 1] = im.img[(int64_t)(y0 + ly) * im.pitch + x0 + lx];
+        }
+    }
+}
+
+// Komura-equivalence labelling of the tile's flat zones (8-connectivity):
+// link every pixel to its first equal predecessor (up, up-left, up-right,
+// left), compress, unite the left and up-right edges, compress.  Every link
+// points to a smaller index, so each zone's root is its minimum raster index.
+// With CLASSIFY the first pass also records the pixel's evidence bits and the
+// cross-seam equality bits of the right column and bottom row.
+template <int T, int NT, bool CLASSIFY>
+__device__ __forceinline__ void tile_label(TileSmem<T>& sm, const ImgDev& im, int x0, int y0,
+                                           int tw, int th) {
+    constexpr int P = T + 2;
+    const bool hasL = x0 > 0, hasR = x0 + tw < im.w, hasU = y0 > 0, hasD = y0 + th < im.h;
+    for (int p = threadIdx.x; p < T * T; p += NT) {
+        const int lx = p % T, ly = p / T;
+        if (lx >= tw || ly >= th) continue;
+        const uint8_t* c = &sm.img[(ly + 1) * P + lx + 1];
+        const uint8_t v = c[0];
+        const uint8_t nu = c[-P], nul = c[-P - 1], nur = c[-P + 1], nl = c[-1];
+        uint32_t parent = p;
+        if (ly > 0 && nu == v) parent = p - T;
+        else if (ly > 0 && lx > 0 && nul == v) parent = p - T - 1;
+        else if (ly > 0 && lx + 1 < tw && nur == v) parent = p - T + 1;
+        else if (lx > 0 && nl == v) parent = p - 1;
+        sm.lab[p] = parent;
+        if (CLASSIFY) {
+            const uint8_t nr = c[1], ndl = c[P - 1], nd = c[P], ndr = c[P + 1];
+            const uint8_t hi = max(max(max(nu, nul), max(nur, nl)), max(max(nr, ndl), max(nd, ndr)));
+            const uint8_t lo = min(min(min(nu, nul), min(nur, nl)), min(min(nr, ndl), min(nd, ndr)));
+            uint32_t f = (hi > v ? F_G : 0u) | (lo < v ? F_S : 0u);
+            const bool perim = lx == 0 || ly == 0 || lx == tw - 1 || ly == th - 1;
+            if (perim) {
+                // equal neighbour outside the tile but inside the image
+                auto seam = [&](int dx, int dy, uint8_t n) -> uint32_t {
+                    const int nx = lx + dx, ny = ly + dy;
+                    const bool out = nx < 0 || nx >= tw || ny < 0 || ny >= th;
+                    const bool in_img = (nx >= 0 || hasL) && (nx < tw || hasR) &&
+                                        (ny >= 0 || hasU) && (ny < th || hasD);
+                    return (out && in_img && n == v) ? 1u : 0u;
+                };
+                const uint32_t sul = seam(-1, -1, nul), su = seam(0, -1, nu), sur = seam(1, -1, nur);
+                const uint32_t sl = seam(-1, 0, nl), sr = seam(1, 0, nr);
+                const uint32_t sdl = seam(-1, 1, ndl), sd = seam(0, 1, nd), sdr = seam(1, 1, ndr);
+                if (sul | su | sur | sl | sr | sdl | sd | sdr) f |= F_SEAM;
+                if (lx == tw - 1) sm.eqr[ly] = (uint8_t)(sur | (sr << 1) | (sdr << 2));
+                if (ly == th - 1) sm.eqb[lx] = (uint8_t)(sdl | (sd << 1) | (sdr << 2));
+            }
+            sm.pflag[p] = (uint8_t)f;
+        }
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < T * T; p += NT) {
+        if (p % T < tw && p / T < th) sm.lab[p] = lfind(sm.lab, p);
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < T * T; p += NT) {
+        const int lx = p % T, ly = p / T;
+        if (lx >= tw || ly >= th) continue;
+        const uint8_t* c = &sm.img[(ly + 1) * P + lx + 1];
+        const uint8_t v = c[0];
+        if (lx > 0 && c[-1] == v) lunion(sm.lab, p, p - 1);
+        if (ly > 0 && lx + 1 < tw && c[-P + 1] == v) lunion(sm.lab, p, p - T + 1);
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < T * T; p += NT) {
+        if (p % T < tw && p / T < th) sm.lab[p] = lfind(sm.lab, p);
+    }
+    __syncthreads();
+}
+
+// Tile stage for one tile (see ss_device.hpp, step 1).
+template <int T, int NT>
+__device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, int64_t gtile,
+                                           const Ws& ws) {
+    constexpr int S = 4 * T - 4;
+    const int64_t lt = gtile - im.tile_begin;
+    const int tx = (int)(lt % im.tiles_x), ty = (int)(lt / im.tiles_x);
+    const int x0 = tx * T, y0 = ty * T;
+    const int tw = min(T, im.w - x0), th = min(T, im.h - y0);
+    if (threadIdx.x == 0) {
+        sm.count = 0;
+        sm.err = 0;
+    }
+    tile_load<T, NT, true>(sm, im, x0, y0, tw, th);
+    __syncthreads();
+    tile_label<T, NT, true>(sm, im, x0, y0, tw, th);
+
+    // OR the pixel evidence into the zone root (skipped when there is none).
+    for (int p = threadIdx.x; p < T * T; p += NT) {
+        if (p % T >= tw || p / T >= th) continue;
+        const uint32_t f = sm.pflag[p];
+        if (f) __hip_atomic_fetch_or(&sm.lab[sm.lab[p] & LMASK], f << FLAG_SHIFT, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+
+    // Number the seam zones and fill their fragment slots.
+    for (int p = threadIdx.x; p < T * T; p += NT) {
+        if (p % T >= tw || p / T >= th) continue;
+        const uint32_t w = sm.lab[p];
+        if ((w & LMASK) != (uint32_t)p || !((w >> FLAG_SHIFT) & F_SEAM)) continue;
+        const uint32_t j = atomicAdd(&sm.count, 1u);
+        if (j >= (uint32_t)S) {
+            sm.err = 1u;
+            continue;
+        }
+        sm.lab[p] = w | ((j + 1) << FRAG_SHIFT);
+        ws.sbits[gtile * S + j] = (uint8_t)((w >> FLAG_SHIFT) & 3u);
+        ws.sroot[gtile * S + j] = (uint16_t)p;
+    }
+    __syncthreads();
+
+    // Masks (final for tile-contained zones, provisional for seam zones) and
+    // the perimeter records of seam pixels.
+    uint8_t* rec = ws.rec + gtile * 4 * T;
+    for (int p = threadIdx.x; p < T * T; p += NT) {
+        const int lx = p % T, ly = p / T;
+        if (lx >= tw || ly >= th) continue;
+        const uint32_t rw = sm.lab[sm.lab[p] & LMASK];
+        const uint32_t zf = rw >> FLAG_SHIFT;
+        const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx;
+        im.mx[o] = (zf & F_G) ? 0 : 255;
+        im.mn[o] = (zf & F_S) ? 0 : 255;
+        if (sm.pflag[p] & F_SEAM) {
+            const uint32_t jr = (rw >> FRAG_SHIFT) & 0xFFu;
+            if (jr == 0u) {
+                sm.err = 2u;
+                continue;
+            }
+            const uint8_t j = (uint8_t)(jr - 1u);
+            if (ly == 0) rec[lx] = j;
+            if (ly == th - 1) rec[T + lx] = j;
+            if (lx == 0) rec[2 * T + ly] = j;
+            if (lx == tw - 1) rec[3 * T + ly] = j;
+        }
+    }
+    __syncthreads();
+    const uint32_t count = sm.count;
+    if (count) {
+        uint8_t* eq = ws.eq + gtile * 2 * T;
+        for (int k = threadIdx.x; k < T; k += NT) {
+            eq[k] = (k < th && tw == T) ? sm.eqr[k] : 0;
+            eq[T + k] = (k < tw && th == T) ? sm.eqb[k] : 0;
+        }
+    }
+    if (threadIdx.x == 0) ws.counts[gtile] = min(count, (uint32_t)S) | (sm.err << COUNT_ERR_SHIFT);
+}
+
+// Plain form: one workgroup per tile, XCD-aware tile order.
+template <int T, int NT, bool MULTI>
+__global__ __launch_bounds__(NT) void k_tile_stage(Images ims, Ws ws) {
+    __shared__ TileSmem<T> sm;
+    const int64_t gtile = xcd_block(blockIdx.x, gridDim.x);
+    if (gtile >= ws.ntiles) return;
+    const ImgDev im = locate<MULTI>(ims, gtile);
+    tile_stage<T, NT>(sm, im, gtile, ws);
+}
+
+// Persistent form: a fixed grid drains a device-side tile work queue.  The
+// tile range is split into one contiguous queue per XCD; a workgroup drains
+// its own XCD's queue first and then steals from the others, so neighbouring
+// tiles share an L2 while the load still balances across the chip.
+template <int T, int NT, bool MULTI>
+__global__ __launch_bounds__(NT) void k_tile_stage_persistent(Images ims, Ws ws, int chunk) {
+    __shared__ TileSmem<T> sm;
+    const int home = blockIdx.x % NUM_XCD;
+    for (int q = 0; q < NUM_XCD; ++q) {
+        const int xq = (home + q) % NUM_XCD;
+        for (;;) {
+            __syncthreads();
+            if (threadIdx.x == 0) sm.next = atomicAdd(&ws.hdr->queue[xq], (uint32_t)chunk);
+            __syncthreads();
+            const uint32_t start = sm.next;
+            const uint32_t end_q = ws.hdr->queue_end[xq];
+            if (start >= end_q) break;
+            const uint32_t stop = min(start + (uint32_t)chunk, end_q);
+            for (uint32_t t = start; t < stop; ++t) {
+                const ImgDev im = locate<MULTI>(ims, t);
+                tile_stage<T, NT>(sm, im, t, ws);
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// Resets the work queues of the persistent tile stage (graph-safe: a kernel,
+// not a memset node).
+__global__ void k_queue_reset(Ws ws) {
+    const int i = threadIdx.x;
+    if (i < NUM_XCD) {
+        const int64_t n = ws.ntiles;
+        const int64_t b = n * i / NUM_XCD, e = n * (i + 1) / NUM_XCD;
+        ws.hdr->queue[i] = (uint32_t)b;
+        ws.hdr->queue_end[i] = (uint32_t)e;
+    }
+}
+
+// ---- compaction: exclusive scan of the per-tile counts --------------------
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave, uint32_t& total) {
+    constexpr int W = SCAN_BLOCK / 64;
+    const uint32_t inc = wave_incl_scan(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 63) s_wave[wid] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < W; ++i) {
+            const uint32_t t = s_wave[i];
+            s_wave[i] = acc;
+            acc += t;
+        }
+        s_wave[W] = acc;
+    }
+    __syncthreads();
+    total = s_wave[W];
+    const uint32_t r = s_wave[wid] + inc - v;
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(Ws ws) {
+    __shared__ uint32_t s_wave[SCAN_BLOCK / 64 + 1];
+    __shared__ uint32_t s_err;
+    if (threadIdx.x == 0) s_err = 0;
+    const int64_t t0 = (int64_t)blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
+    uint32_t s = 0, e = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        if (t0 + i < ws.ntiles) {
+            const uint32_t c = ws.counts[t0 + i];
+            s += c & COUNT_MASK;
+            e |= c >> COUNT_ERR_SHIFT;
+        }
+    }
+    uint32_t total;
+    block_excl_scan(s, s_wave, total);
+    if (e) atomicOr(&s_err, e);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ws.bsum[blockIdx.x] = total;
+        ws.berr[blockIdx.x] = s_err;
+    }
+}
+
+// Single workgroup: scan of the block sums; also folds the tile-stage
+// invariant word into the run's word and clears the per-run counters.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_top(Ws ws) {
+    __shared__ uint32_t s_wave[SCAN_BLOCK / 64 + 1];
+    __shared__ uint32_t s_err;
+    if (threadIdx.x == 0) s_err = 0;
+    const uint32_t nb = ws.nblocks;
+    uint32_t carry = 0, err = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += SCAN_BLOCK) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint32_t v = i < nb ? ws.bsum[i] : 0u;
+        if (i < nb) err |= ws.berr[i];
+        uint32_t total;
+        const uint32_t ex = block_excl_scan(v, s_wave, total);
+        if (i < nb) ws.bsum[i] = carry + ex;
+        carry += total;
+    }
+    if (err) atomicOr(&s_err, err);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ws.bsum[nb] = carry;
+        WsHeader* h = ws.hdr;
+        h->magic = WS_MAGIC;
+        h->total = carry;
+        h->error = s_err;
+        h->dirty_count = 0;
+        h->retries = 0;
+        h->tiles = (uint64_t)ws.ntiles;
+        h->tile = ws.tile;
+        h->threads = ws.threads;
+    }
+}
+
+// Per scan block: tile bases, then the dense union-find arrays of the block's
+// fragments (one wave per tile, lanes over the tile's fragments).
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(Ws ws) {
+    __shared__ uint32_t s_wave[SCAN_BLOCK / 64 + 1];
+    __shared__ uint32_t s_base[SCAN_CHUNK];
+    __shared__ uint32_t s_cnt[SCAN_CHUNK];
+    const int64_t tb = (int64_t)blockIdx.x * SCAN_CHUNK;
+    const int64_t t0 = tb + threadIdx.x * SCAN_ITEMS;
+    uint32_t c[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        c[i] = (t0 + i < ws.ntiles) ? (ws.counts[t0 + i] & COUNT_MASK) : 0u;
+        s += c[i];
+    }
+    uint32_t total;
+    uint32_t run = block_excl_scan(s, s_wave, total) + ws.bsum[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        const int li = threadIdx.x * SCAN_ITEMS + i;
+        s_base[li] = run;
+        s_cnt[li] = c[i];
+        if (t0 + i < ws.ntiles) ws.base[t0 + i] = run;
+        run += c[i];
+    }
+    __syncthreads();
+    const int S = ws.slots;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int li = wid; li < SCAN_CHUNK; li += SCAN_BLOCK / 64) {
+        const uint32_t cnt = s_cnt[li];
+        if (!cnt) continue;
+        const uint32_t b = s_base[li];
+        const int64_t slot0 = (tb + li) * S;
+        for (uint32_t j = lane; j < cnt; j += 64) {
+            ws.dparent[b + j] = b + j;
+            ws.dbits[b + j] = ws.sbits[slot0 + j];
+        }
+    }
+}
+
+// ---- seam merge -----------------------------------------------------------
+
+// One thread per (tile, k): the tile's right-column pixel k against the three
+// pixels right of it, and its bottom-row pixel k against the three below it.
+// Together these visit every equal pixel pair that straddles a tile seam.
+template <int T, bool MULTI>
+__global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t gtile = g / T;
+    const int k = (int)(g % T);
+    uint32_t retries = 0;
+    bool bad = false;
+    const uint32_t cnt = gtile < ws.ntiles ? (ws.counts[gtile] & COUNT_MASK) : 0u;
+    if (cnt != 0) {
+        const ImgDev im = locate<MULTI>(ims, gtile);
+        const int64_t lt = gtile - im.tile_begin;
+        const int tx = (int)(lt % im.tiles_x), ty = (int)(lt / im.tiles_x);
+        const uint8_t* eq = ws.eq + gtile * 2 * T;
+        const uint8_t* rec = ws.rec + gtile * 4 * T;
+        const uint32_t base = ws.base[gtile];
+        if (tx + 1 < im.tiles_x) {
+            const uint32_t e = eq[k];
+            if (e) {
+                const uint32_t ja = rec[3 * T + k];
+                const uint32_t fa = base + ja;
+                const int y = ty * T + k;
+                for (int d = 0; d < 3; ++d) {
+                    if (!(e & (1u << d))) continue;
+                    const int ny = y + d - 1;
+                    const int nty = ny / T;
+                    const int64_t nt = im.tile_begin + (int64_t)nty * im.tiles_x + tx + 1;
+                    const uint32_t jb = ws.rec[nt * 4 * T + 2 * T + (ny - nty * T)];
+                    if (ja >= cnt || jb >= (ws.counts[nt] & COUNT_MASK)) { bad = true; continue; }
+                    gunion(ws.dparent, fa, ws.base[nt] + jb, retries);
+                }
+            }
+        }
+        if (ty + 1 < im.tiles_y) {
+            const uint32_t e = eq[T + k];
+            if (e) {
+                const uint32_t ja = rec[T + k];
+                const uint32_t fa = base + ja;
+                const int x = tx * T + k;
+                for (int d = 0; d < 3; ++d) {
+                    if (!(e & (1u << d))) continue;
+                    const int nx = x + d - 1;
+                    const int ntx = nx / T;
+                    const int64_t nt = im.tile_begin + (int64_t)(ty + 1) * im.tiles_x + ntx;
+                    const uint32_t jb = ws.rec[nt * 4 * T + (nx - ntx * T)];
+                    if (ja >= cnt || jb >= (ws.counts[nt] & COUNT_MASK)) { bad = true; continue; }
+                    gunion(ws.dparent, fa, ws.base[nt] + jb, retries);
+                }
+            }
+        }
+    }
+    const uint32_t r = wave_sum(retries);
+    if ((threadIdx.x & 63) == 0 && r) atomicAdd((unsigned long long*)&ws.hdr->retries, (unsigned long long)r);
+    // a perimeter record out of range would mean a broken tile stage: report
+    // it instead of following it
+    if (bad) atomicOr(&ws.hdr->error, 2u);
+}
+
+// ---- resolve: compress + OR local evidence into the roots ----------------
+
+__global__ __launch_bounds__(256) void k_resolve(Ws ws) {
+    const uint32_t total = ws.hdr->total;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const uint32_t r = gfind(ws.dparent, i);
+        if (r != i) g_store(ws.dparent + i, r);
+        const uint32_t lb = ws.dbits[i] & 3u;
+        if (lb) __hip_atomic_fetch_or(ws.dbits + r, lb << 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---- mark: fragments whose output changes, dirty-tile list ---------------
+
+__global__ __launch_bounds__(256) void k_mark(Ws ws) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool dirty = false;
+    if (t < ws.ntiles) {
+        const uint32_t cnt = ws.counts[t] & COUNT_MASK;
+        const uint32_t b = ws.base[t];
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const uint32_t i = b + j;
+            const uint32_t own = ws.dbits[i];
+            const uint32_t res = (ws.dbits[ws.dparent[i]] >> 2) & 3u;
+            if (res & ~own & 3u) {
+                ws.dbits[i] = own | DB_CHANGED;
+                dirty = true;
+            }
+        }
+    }
+    const uint64_t m = __ballot(dirty);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t slot0 = 0;
+    if (lane == leader) slot0 = atomicAdd(&ws.hdr->dirty_count, (uint32_t)__popcll(m));
+    slot0 = __shfl(slot0, leader, 64);
+    if (dirty) {
+        const uint32_t rank = __popcll(m & ((1ull << lane) - 1ull));
+        ws.dirty[slot0 + rank] = (uint32_t)t;
+    }
+}
+
+// ---- write-back of changed seam zones ------------------------------------
+
+template <int T, int NT, bool MULTI>
+__global__ __launch_bounds__(NT) void k_writeback(Images ims, Ws ws) {
+    __shared__ TileSmem<T> sm;
+    constexpr int S = 4 * T - 4;
+    const uint32_t nd = ws.hdr->dirty_count;
+    for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
+        const int64_t gtile = ws.dirty[d];
+        const ImgDev im = locate<MULTI>(ims, gtile);
+        const int64_t lt = gtile - im.tile_begin;
+        const int tx = (int)(lt % im.tiles_x), ty = (int)(lt / im.tiles_x);
+        const int x0 = tx * T, y0 = ty * T;
+        const int tw = min(T, im.w - x0), th = min(T, im.h - y0);
+        for (int p = threadIdx.x; p < T * T; p += NT) sm.pflag[p] = 0;
+        tile_load<T, NT, false>(sm, im, x0, y0, tw, th);
+        __syncthreads();
+        tile_label<T, NT, false>(sm, im, x0, y0, tw, th);
+        const uint32_t cnt = ws.counts[gtile] & COUNT_MASK;
+        const uint32_t b = ws.base[gtile];
+        for (uint32_t j = threadIdx.x; j < cnt; j += NT) {
+            if (ws.dbits[b + j] & DB_CHANGED) sm.pflag[ws.sroot[gtile * S + j]] = (uint8_t)(j + 1);
+        }
+        __syncthreads();
+        for (int p = threadIdx.x; p < T * T; p += NT) {
+            const int lx = p % T, ly = p / T;
+            if (lx >= tw || ly >= th) continue;
+            const uint32_t j1 = sm.pflag[sm.lab[p]];
+            if (!j1) continue;
+            const uint32_t res = (ws.dbits[ws.dparent[b + j1 - 1]] >> 2) | ws.dbits[b + j1 - 1];
+            const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx;
+            im.mx[o] = (res & F_G) ? 0 : 255;
+            im.mn[o] = (res & F_S) ? 0 : 255;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- host sequencer -------------------------------------------------------
+
+template <int T, int NT, bool MULTI>
+static void launch_tile_kernels(const Images& ims, const Ws& ws, bool persistent, int grid_cap,
+                                hipStream_t s) {
+    if (persistent) {
+        k_queue_reset<<<1, 64, 0, s>>>(ws);
+        const int64_t grid = std::min<int64_t>(ws.ntiles, grid_cap);
+        // claims of several tiles keep the queue atomics off the critical path
+        const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(32, ws.ntiles / (grid * 16)));
+        k_tile_stage_persistent<T, NT, MULTI><<<(unsigned)grid, NT, 0, s>>>(ims, ws, chunk);
+    } else {
+        k_tile_stage<T, NT, MULTI><<<(unsigned)ws.ntiles, NT, 0, s>>>(ims, ws);
+    }
+}
+
+template <int T, int NT, bool MULTI>
+static void launch_writeback(const Images& ims, const Ws& ws, int grid_cap, hipStream_t s) {
+    const int64_t grid = std::min<int64_t>(ws.ntiles, grid_cap);
+    k_writeback<T, NT, MULTI><<<(unsigned)grid, NT, 0, s>>>(ims, ws);
+}
+
+template <int T, bool MULTI>
+static void launch_merge(const Images& ims, const Ws& ws, hipStream_t s) {
+    const int64_t merge_threads = ws.ntiles * T;
+    const unsigned merge_grid = (unsigned)((merge_threads + 255) / 256);
+    k_seam_merge<T, MULTI><<<merge_grid, 256, 0, s>>>(ims, ws);
+}
+
+#define SSX_DISPATCH(T_, NT_, CALL)                                        \
+    switch ((T_) * 1000 + (NT_)) {                                         \
+    case 16064: CALL(16, 64); break;                                       \
+    case 16128: CALL(16, 128); break;                                      \
+    case 16256: CALL(16, 256); break;                                      \
+    case 32064: CALL(32, 64); break;                                       \
+    case 32128: CALL(32, 128); break;                                      \
+    case 32256: CALL(32, 256); break;                                      \
+    case 32512: CALL(32, 512); break;                                      \
+    case 64064: CALL(64, 64); break;                                       \
+    case 64128: CALL(64, 128); break;                                      \
+    case 64256: CALL(64, 256); break;                                      \
+    case 64512: CALL(64, 512); break;                                      \
+    default: return hipErrorInvalidValue;                                  \
+    }
+
+template <bool MULTI>
+static hipError_t launch_all(const Images& ims, const Ws& ws, int T, int NT, bool persistent,
+                             int cus, hipStream_t s) {
+    const int grid_cap = std::max(1, cus) * 8;
+    const ProfileHook& hook = profile_hook();
+    if (hook.tile_begin) (void)hipEventRecord(hook.tile_begin, s);
+#define CALL_TILE(a, b) launch_tile_kernels<a, b, MULTI>(ims, ws, persistent, grid_cap, s)
+    SSX_DISPATCH(T, NT, CALL_TILE)
+#undef CALL_TILE
+    if (hook.tile_end) (void)hipEventRecord(hook.tile_end, s);
+    const uint32_t nb = ws.nblocks;
+    k_scan_reduce<<<nb, SCAN_BLOCK, 0, s>>>(ws);
+    k_scan_top<<<1, SCAN_BLOCK, 0, s>>>(ws);
+    k_scan_apply<<<nb, SCAN_BLOCK, 0, s>>>(ws);
+    switch (T) {
+    case 16: launch_merge<16, MULTI>(ims, ws, s); break;
+    case 32: launch_merge<32, MULTI>(ims, ws, s); break;
+    case 64: launch_merge<64, MULTI>(ims, ws, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    const int64_t max_frag = ws.ntiles * (int64_t)ws.slots;
+    const unsigned resolve_grid =
+        (unsigned)std::max<int64_t>(1, std::min<int64_t>((max_frag + 255) / 256, (int64_t)grid_cap * 2));
+    k_resolve<<<resolve_grid, 256, 0, s>>>(ws);
+    k_mark<<<(unsigned)((ws.ntiles + 255) / 256), 256, 0, s>>>(ws);
+#define CALL_WB(a, b) launch_writeback<a, b, MULTI>(ims, ws, grid_cap, s)
+    SSX_DISPATCH(T, NT, CALL_WB)
+#undef CALL_WB
+    return hipGetLastError();
+}
+
+hipError_t launch_pipeline(const Images& ims, const Ws& ws, int T, int NT, bool persistent,
+                           int cus, hipStream_t s) {
+    if (ims.count > 1) return launch_all<true>(ims, ws, T, NT, persistent, cus, s);
+    return launch_all<false>(ims, ws, T, NT, persistent, cus, s);
+}
+
+}  // namespace ssx

@@ -1,0 +1,237 @@
+"""ctypes binding of libss_extrema.so (the C ABI in include/ss_extrema.h).
+
+PyTorch is used only for device memory and streams.  Every call goes through
+the HIP library; there is no CPU fallback: if the library or a gfx950 device is
+missing the calls raise.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libss_extrema.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "ss_extrema.h")
+
+SS_OK, SS_ERR_ARG, SS_ERR_WORKSPACE, SS_ERR_HIP, SS_ERR_INVARIANT, SS_ERR_CAPACITY, SS_ERR_NO_DEVICE = range(7)
+TILES = (16, 32, 64)
+THREADS = (64, 128, 256, 512)
+CONFIGS = tuple((t, n) for t in TILES for n in THREADS if n <= t * t)
+
+
+class SSError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        super().__init__("%s: %s (status %d)" % (what, _lib().ss_status_str(status).decode(), status))
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("tile", ctypes.c_int), ("threads", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("invariant", ctypes.c_uint32), ("dirty_tiles", ctypes.c_uint32),
+                ("fragments", ctypes.c_uint64), ("dsu_retries", ctypes.c_uint64),
+                ("tiles", ctypes.c_uint64)]
+
+
+class ImageDesc(ctypes.Structure):
+    _fields_ = [("img", ctypes.c_void_p), ("max", ctypes.c_void_p), ("min", ctypes.c_void_p),
+                ("pitch", ctypes.c_uint64), ("out_pitch", ctypes.c_uint64),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32)]
+
+
+_LIB = None
+
+
+def _lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libss_extrema.so is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, sz, i = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+    cfgp = ctypes.POINTER(Config)
+    lib.ss_status_str.argtypes = [i]
+    lib.ss_status_str.restype = ctypes.c_char_p
+    lib.ss_workspace_size.argtypes = [i, i, cfgp, ctypes.POINTER(sz)]
+    lib.ss_extrema_u8.argtypes = [vp, sz, i, i, vp, vp, sz, vp, sz, cfgp, vp]
+    lib.ss_read_stats.argtypes = [vp, sz, ctypes.POINTER(Stats), vp]
+    lib.ss_read_tile_counts.argtypes = [vp, sz, ctypes.POINTER(ctypes.c_uint32), sz, vp]
+    descp = ctypes.POINTER(ImageDesc)
+    lib.ss_batch_workspace_size.argtypes = [descp, i, cfgp, ctypes.POINTER(sz)]
+    lib.ss_extrema_batch_u8.argtypes = [descp, i, vp, sz, cfgp, vp]
+    lib.ss_batch_plan_create.argtypes = [descp, i, vp, sz, cfgp, vp, ctypes.POINTER(vp)]
+    lib.ss_batch_plan_run.argtypes = [vp, vp]
+    lib.ss_batch_plan_destroy.argtypes = [vp]
+    lib.ss_autotune.argtypes = [descp, i, i, vp, cfgp, ctypes.POINTER(ctypes.c_double)]
+    lib.ss_profile_events.argtypes = [vp, vp]
+    lib.ss_set_default_config.argtypes = [cfgp]
+    lib.ss_get_default_config.argtypes = [cfgp]
+    for name in ("ss_workspace_size", "ss_extrema_u8", "ss_read_stats", "ss_read_tile_counts",
+                 "ss_batch_workspace_size", "ss_extrema_batch_u8", "ss_batch_plan_create",
+                 "ss_batch_plan_run", "ss_batch_plan_destroy", "ss_autotune",
+                 "ss_set_default_config", "ss_get_default_config", "ss_profile_events"):
+        getattr(lib, name).restype = i
+    _LIB = lib
+    return lib
+
+
+def lib():
+    return _lib()
+
+
+def _check(status, what):
+    if status != SS_OK:
+        raise SSError(status, what)
+
+
+def _cfg(tile, threads):
+    if tile is None and threads is None:
+        return None
+    c = Config()
+    d = default_config()
+    c.tile = d[0] if tile is None else tile
+    c.threads = d[1] if threads is None else threads
+    return ctypes.byref(c)
+
+
+def default_config():
+    c = Config()
+    _check(_lib().ss_get_default_config(ctypes.byref(c)), "ss_get_default_config")
+    return c.tile, c.threads
+
+
+def set_default_config(tile, threads):
+    c = Config(tile, threads)
+    _check(_lib().ss_set_default_config(ctypes.byref(c)), "ss_set_default_config")
+
+
+def workspace_size(width, height, tile=None, threads=None):
+    out = ctypes.c_size_t()
+    _check(_lib().ss_workspace_size(width, height, _cfg(tile, threads), ctypes.byref(out)),
+           "ss_workspace_size")
+    return out.value
+
+
+def _stream_handle(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def new_workspace(nbytes, device="cuda"):
+    """Device workspace as a uint8 tensor (256-byte aligned by the allocator)."""
+    import torch
+    ws = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+    assert ws.data_ptr() % 256 == 0
+    return ws
+
+
+def extrema(img, tile=None, threads=None, ws=None, out=None, stream=None):
+    """Dual regional extrema of a 2-D uint8 CUDA tensor (rows may be strided).
+    Returns (max_mask, min_mask, ws); masks are uint8 0/255 tensors."""
+    import torch
+    if img.dim() != 2 or img.dtype != torch.uint8 or not img.is_cuda or img.stride(1) != 1:
+        raise ValueError("img must be a 2-D row-major uint8 CUDA tensor")
+    h, w = img.shape
+    need = workspace_size(w, h, tile, threads)
+    if ws is None or ws.numel() < need:
+        ws = new_workspace(need, img.device)
+    if out is None:
+        mx = torch.empty((h, w), dtype=torch.uint8, device=img.device)
+        mn = torch.empty((h, w), dtype=torch.uint8, device=img.device)
+    else:
+        mx, mn = out
+    assert mx.stride() == mn.stride() and mx.stride(1) == 1
+    _check(_lib().ss_extrema_u8(img.data_ptr(), img.stride(0), w, h, mx.data_ptr(), mn.data_ptr(),
+                                mx.stride(0), ws.data_ptr(), ws.numel(), _cfg(tile, threads),
+                                _stream_handle(stream)), "ss_extrema_u8")
+    return mx, mn, ws
+
+
+def stats(ws, stream=None, check=True):
+    st = Stats()
+    status = _lib().ss_read_stats(ws.data_ptr(), ws.numel(), ctypes.byref(st), _stream_handle(stream))
+    if check:
+        _check(status, "ss_read_stats")
+    return {"status": status, "invariant": st.invariant, "dirty_tiles": st.dirty_tiles,
+            "fragments": st.fragments, "dsu_retries": st.dsu_retries, "tiles": st.tiles}
+
+
+def tile_counts(ws, tiles_y, tiles_x, stream=None):
+    import numpy as np
+    out = np.zeros(tiles_y * tiles_x, dtype=np.uint32)
+    _check(_lib().ss_read_tile_counts(ws.data_ptr(), ws.numel(),
+                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), out.size,
+                                      _stream_handle(stream)), "ss_read_tile_counts")
+    return out.reshape(tiles_y, tiles_x)
+
+
+def descs(images, outs):
+    """ImageDesc array for lists of input tensors and (max, min) output pairs."""
+    arr = (ImageDesc * len(images))()
+    for k, (img, (mx, mn)) in enumerate(zip(images, outs)):
+        h, w = img.shape
+        arr[k].img = img.data_ptr()
+        arr[k].max = mx.data_ptr()
+        arr[k].min = mn.data_ptr()
+        arr[k].pitch = img.stride(0)
+        arr[k].out_pitch = mx.stride(0)
+        arr[k].width = w
+        arr[k].height = h
+    return arr
+
+
+def batch_workspace_size(desc_array, tile=None, threads=None):
+    out = ctypes.c_size_t()
+    _check(_lib().ss_batch_workspace_size(desc_array, len(desc_array), _cfg(tile, threads),
+                                          ctypes.byref(out)), "ss_batch_workspace_size")
+    return out.value
+
+
+def extrema_batch(desc_array, ws, tile=None, threads=None, stream=None):
+    _check(_lib().ss_extrema_batch_u8(desc_array, len(desc_array), ws.data_ptr(), ws.numel(),
+                                      _cfg(tile, threads), _stream_handle(stream)),
+           "ss_extrema_batch_u8")
+
+
+class BatchPlan:
+    """hipGraph-captured batch pipeline (ss_batch_plan_*)."""
+
+    def __init__(self, desc_array, ws, tile=None, threads=None, stream=None):
+        self._h = ctypes.c_void_p()
+        self.ws = ws
+        self.descs = desc_array
+        _check(_lib().ss_batch_plan_create(desc_array, len(desc_array), ws.data_ptr(), ws.numel(),
+                                           _cfg(tile, threads), _stream_handle(stream),
+                                           ctypes.byref(self._h)), "ss_batch_plan_create")
+
+    def run(self, stream=None):
+        _check(_lib().ss_batch_plan_run(self._h, _stream_handle(stream)), "ss_batch_plan_run")
+
+    def close(self):
+        if self._h:
+            _lib().ss_batch_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def profile_events(begin=None, end=None):
+    """Route the tile-stage kernel through a pair of torch.cuda.Event objects
+    (recorded on the pipeline's own stream); None, None turns it off."""
+    b = ctypes.c_void_p(begin.cuda_event) if begin is not None else None
+    e = ctypes.c_void_p(end.cuda_event) if end is not None else None
+    _check(_lib().ss_profile_events(b, e), "ss_profile_events")
+
+
+def autotune(desc_array, reps=5, stream=None):
+    best = Config()
+    rate = ctypes.c_double()
+    _check(_lib().ss_autotune(desc_array, len(desc_array), reps, _stream_handle(stream),
+                              ctypes.byref(best), ctypes.byref(rate)), "ss_autotune")
+    return (best.tile, best.threads), rate.value

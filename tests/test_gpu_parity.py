@@ -1,0 +1,164 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bit-exact masks are required everywhere: the 206 golden triples, random
+images at every (tile, threads) config, the seeded benchmark configurations
+(oracle comparison where the oracle finishes in seconds, size-independent
+properties beyond that), and the per-tile seam-fragment counts.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from sscommon import golden_cases, oracle_global_uf, oracle_sparse_seam
+from ssx import sparse_seam as ss
+from ssx import ssgen
+
+pytestmark = pytest.mark.gpu
+
+CASES = list(golden_cases())
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run(img_np, tile=None, threads=None, ws=None):
+    img = torch.as_tensor(np.ascontiguousarray(img_np)).cuda()
+    mx, mn, ws = ss.extrema(img, tile, threads, ws)
+    torch.cuda.synchronize()
+    st = ss.stats(ws)
+    assert st["invariant"] == 0
+    return mx.cpu().numpy(), mn.cpu().numpy(), ws, st
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert torch.cuda.is_available()
+    return torch.device("cuda")
+
+
+@pytest.mark.parametrize("tile,threads", ss.CONFIGS)
+def test_golden_fixtures(dev, tile, threads):
+    say("golden fixtures tile=%d threads=%d" % (tile, threads))
+    ws = None
+    for suite, name, img, emax, emin, _ in CASES:
+        mx, mn, ws, _ = run(img, tile, threads, ws)
+        assert np.array_equal(mx, emax), (suite, name)
+        assert np.array_equal(mn, emin), (suite, name)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_images_all_configs(dev, seed):
+    rng = np.random.default_rng(1000 + seed)
+    h, w = int(rng.integers(1, 300)), int(rng.integers(1, 300))
+    kind = seed % 4
+    if kind == 0:
+        img = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    elif kind == 1:
+        b = int(rng.integers(2, 40))
+        img = np.repeat(np.repeat(rng.integers(0, 4, (h // b + 1, w // b + 1), dtype=np.uint8), b, 0), b, 1)[:h, :w]
+    elif kind == 2:
+        img = ssgen.smooth(h, w, float(rng.choice([1.0, 3.0])), int(rng.choice([2, 4, 16])), seed).numpy()
+    else:
+        img = ssgen.checker(h, w, cell=int(rng.integers(1, 9))).numpy()
+    say("random seed=%d %dx%d kind=%d" % (seed, h, w, kind))
+    emax, emin = oracle_global_uf(img, threads=4)
+    for tile, threads in ss.CONFIGS:
+        mx, mn, _, _ = run(img, tile, threads)
+        assert np.array_equal(mx, emax), (tile, threads)
+        assert np.array_equal(mn, emin), (tile, threads)
+
+
+@pytest.mark.parametrize("tile", ss.TILES)
+def test_tile_counts_match_oracle(dev, tile):
+    rng = np.random.default_rng(5)
+    for kind in range(3):
+        h, w = 197, 263
+        if kind == 0:
+            img = rng.integers(0, 256, (h, w), dtype=np.uint8)
+        elif kind == 1:
+            img = ssgen.smooth(h, w, 4.0, 8, 3).numpy()
+        else:
+            img = ssgen.rects(h, w, 1, tile // 2).numpy()
+        say("tile counts tile=%d kind=%d" % (tile, kind))
+        mx, mn, ws, st = run(img, tile, 256 if tile > 16 else 128)
+        omx, omn, counts = oracle_sparse_seam(img, tile)
+        assert np.array_equal(mx, omx) and np.array_equal(mn, omn)
+        got = ss.tile_counts(ws, *counts.shape)
+        assert np.array_equal(got, counts.astype(np.uint32))
+        assert st["fragments"] == int(counts.sum())
+
+
+def test_config1_image(dev):
+    img = ssgen.config1()
+    say("config1 512x512 sigma2 q32 seed0")
+    emax, emin = oracle_global_uf(img.numpy())
+    for tile, threads in ((16, 256), (32, 256), (64, 256)):
+        mx, mn, _, _ = run(img.numpy(), tile, threads)
+        assert np.array_equal(mx, emax) and np.array_equal(mn, emin)
+
+
+@pytest.mark.parametrize("side", [512, 1024, 2048])
+def test_sweep_small_sides(dev, side):
+    for seed in (0, 1):
+        for name, img in ssgen.sweep_images(side, seed, device="cuda"):
+            say("sweep side=%d seed=%d %s" % (side, seed, name))
+            emax, emin = oracle_global_uf(img.cpu().numpy())
+            for tile in ss.TILES:
+                mx, mn, _, _ = run(img.cpu().numpy(), tile, 256)
+                assert np.array_equal(mx, emax), (name, tile)
+                assert np.array_equal(mn, emin), (name, tile)
+
+
+def test_adversarial_4096(dev):
+    for tile in ss.TILES:
+        for name, img in ssgen.adversarial_images(4096, tile, 0, device="cuda"):
+            say("adversarial 4096 tile=%d %s" % (tile, name))
+            h = img.cpu().numpy()
+            emax, emin = oracle_global_uf(h)
+            mx, mn, ws, st = run(h, tile, 256)
+            assert np.array_equal(mx, emax), (name, tile)
+            assert np.array_equal(mn, emin), (name, tile)
+            if name == "constant":
+                assert (mx == 255).all() and (mn == 255).all()
+            if name.startswith("serpentine"):
+                # the whole path is one zone with a greater neighbour at its start
+                assert not (mx[h == 254] == 255).any()
+
+
+def test_large_images_vs_global_uf(dev):
+    for side in (4096, 8192):
+        for name, img in ssgen.sweep_images(side, 0, device="cuda", sigmas=(4,), levels=(2, 16)):
+            say("large side=%d %s" % (side, name))
+            h = img.cpu().numpy()
+            emax, emin = oracle_global_uf(h)
+            mx, mn, _, _ = run(h, 32, 256)
+            assert np.array_equal(mx, emax) and np.array_equal(mn, emin), name
+
+
+def test_strided_rows_and_outputs(dev):
+    rng = np.random.default_rng(9)
+    base = torch.as_tensor(rng.integers(0, 4, (130, 300), dtype=np.uint8)).cuda()
+    view = base[:, 17:17 + 211]            # pitch 300, width 211
+    omx = torch.full((130, 256), 7, dtype=torch.uint8, device="cuda")
+    omn = torch.full((130, 256), 7, dtype=torch.uint8, device="cuda")
+    mx, mn, ws = ss.extrema(view, 32, 256, out=(omx[:, :211], omn[:, :211]))
+    torch.cuda.synchronize()
+    emax, emin = oracle_global_uf(view.cpu().numpy())
+    assert np.array_equal(mx.cpu().numpy(), emax) and np.array_equal(mn.cpu().numpy(), emin)
+    assert (omx[:, 211:] == 7).all() and (omn[:, 211:] == 7).all()
+
+
+def test_repeat_runs_are_identical(dev):
+    img = ssgen.smooth(1000, 1500, 4.0, 4, 2, device="cuda")
+    ref = None
+    ws = None
+    for _ in range(5):
+        mx, mn, ws = ss.extrema(img, 32, 256, ws)
+        cur = (mx.cpu().numpy(), mn.cpu().numpy())
+        if ref is None:
+            ref = cur
+        assert np.array_equal(ref[0], cur[0]) and np.array_equal(ref[1], cur[1])
