@@ -154,6 +154,70 @@ ss_status ss_set_default_config(const ss_config* cfg);
 ss_status ss_get_default_config(ss_config* cfg);
 
 /* ------------------------------------------------------------------------ */
+/* Serving: a continuous-batching request loop.  No reference counterpart   */
+/* (the reference processes one image at a time, gpu_common_v0_6.cu:584).   */
+/*                                                                          */
+/* Requests carry host images.  A scheduler thread admits pending requests  */
+/* in arrival order while their device footprint (image + two masks + their */
+/* share of the tile / seam-DSU workspace) fits under the HBM cap, folds    */
+/* every admitted request into the next batched launch, and keeps up to     */
+/* `streams` launches in flight, so one batch's copies overlap another's    */
+/* kernels.  Requests that arrive while batches run join the next launch.   */
+/* The device arena is allocated once by ss_server_create.                  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct ss_server ss_server;
+
+typedef struct ss_server_config {
+    uint64_t hbm_cap_bytes;  /* device arena = admission cap (bytes)            */
+    int max_batch;           /* requests per launch, 0 = no limit               */
+    int streams;             /* launches in flight, >= 1 (default 2)            */
+    int tile;                /* 0 = default config                               */
+    int threads;
+} ss_server_config;
+
+typedef struct ss_request {
+    const uint8_t* host_img; /* width x height, row-major, pitch = width        */
+    uint8_t* host_max;       /* outputs, same layout                            */
+    uint8_t* host_min;
+    int32_t width;
+    int32_t height;
+    uint64_t tag;            /* caller's value, returned in the completion      */
+} ss_request;
+
+typedef struct ss_completion {
+    uint64_t id;
+    uint64_t tag;
+    int32_t status;          /* ss_status of the request                        */
+    int32_t batch_size;      /* requests in the launch that carried it          */
+    double submit_s;         /* host steady-clock seconds                       */
+    double launch_s;
+    double done_s;
+} ss_completion;
+
+typedef struct ss_server_stats {
+    uint64_t submitted, completed, rejected, launches;
+    uint64_t pixels;
+    uint64_t peak_inflight_bytes;
+} ss_server_stats;
+
+ss_status ss_server_create(const ss_server_config* cfg, ss_server** server);
+/* Non-blocking.  SS_ERR_CAPACITY if the request alone exceeds the cap. */
+ss_status ss_server_submit(ss_server* s, const ss_request* req, uint64_t* id);
+/* Blocks until request `id` completes and returns its record. */
+ss_status ss_server_wait(ss_server* s, uint64_t id, ss_completion* out);
+/* Blocks until every submitted request has completed. */
+ss_status ss_server_drain(ss_server* s);
+ss_status ss_server_get_stats(ss_server* s, ss_server_stats* out);
+ss_status ss_server_destroy(ss_server* s);
+
+/* Open-loop load generator: submits reqs[i] at Poisson arrival times of rate
+ * `qps` (seeded), drains, and writes one completion per request into out[i]
+ * (host clock, so latency = done_s - submit_s). */
+ss_status ss_server_run_open_loop(ss_server* s, const ss_request* reqs, int n, double qps,
+                                  uint64_t seed, ss_completion* out);
+
+/* ------------------------------------------------------------------------ */
 /* Measurement aid (no reference counterpart; the reference times the whole */
 /* kernel sequence with one cudaEvent pair, gpu_common_v0_6.cu:338-347).    */
 /* ------------------------------------------------------------------------ */

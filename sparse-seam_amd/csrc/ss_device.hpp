@@ -96,6 +96,7 @@ struct Ws {
     uint32_t* dparent;   // [tiles * S] dense union-find parents
     uint32_t* dbits;     // [tiles * S] bits 0-1 local, 2-3 resolved, 4 changed
     uint32_t* dirty;     // [tiles] dirty tile list
+    uint32_t* tile_img;  // [tiles] image index of each tile (batches only)
     int64_t ntiles;
     uint32_t nblocks;    // scan blocks = ceil(ntiles / SCAN_CHUNK)
     int tile;            // T
@@ -107,18 +108,13 @@ constexpr uint32_t WS_MAGIC = 0x53535831u;  // "SSX1"
 
 // Image of a global tile id.  MULTI is a template parameter so that the
 // single-image kernels read the descriptor straight from the kernel
-// arguments and never form a pointer that could alias either source.
+// arguments; batched kernels look it up through the per-tile image index.
 template <bool MULTI>
-__device__ __forceinline__ ImgDev locate(const Images& ims, int64_t tile) {
+__device__ __forceinline__ ImgDev locate(const Images& ims, const uint32_t* tile_img, int64_t tile) {
     if constexpr (!MULTI) {
         return ims.one;
     } else {
-        int lo = 0, hi = ims.count - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (ims.table[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
-        }
-        return ims.table[lo];
+        return ims.table[tile_img[tile]];
     }
 }
 
@@ -181,6 +177,18 @@ __device__ __forceinline__ uint32_t gfind(uint32_t* parent, uint32_t x) {
         if (gp == p) return p;
         g_store(parent + x, gp);
         x = gp;
+    }
+}
+
+// Read-only find, for full compression after all unions: every thread then
+// writes only its own parent entry (with the root), so no halving store of
+// another thread can overwrite a root with a mere ancestor.
+__device__ __forceinline__ uint32_t gfind_ro(const uint32_t* parent, uint32_t x) {
+    uint32_t r = g_load(parent + x);
+    for (;;) {
+        const uint32_t n = g_load(parent + r);
+        if (n == r) return r;
+        r = n;
     }
 }
 

@@ -205,7 +205,7 @@ __global__ __launch_bounds__(NT) void k_tile_stage(Images ims, Ws ws) {
     __shared__ TileSmem<T> sm;
     const int64_t gtile = xcd_block(blockIdx.x, gridDim.x);
     if (gtile >= ws.ntiles) return;
-    const ImgDev im = locate<MULTI>(ims, gtile);
+    const ImgDev im = locate<MULTI>(ims, ws.tile_img, gtile);
     tile_stage<T, NT>(sm, im, gtile, ws);
 }
 
@@ -228,12 +228,19 @@ __global__ __launch_bounds__(NT) void k_tile_stage_persistent(Images ims, Ws ws,
             if (start >= end_q) break;
             const uint32_t stop = min(start + (uint32_t)chunk, end_q);
             for (uint32_t t = start; t < stop; ++t) {
-                const ImgDev im = locate<MULTI>(ims, t);
+                const ImgDev im = locate<MULTI>(ims, ws.tile_img, t);
                 tile_stage<T, NT>(sm, im, t, ws);
                 __syncthreads();
             }
         }
     }
+}
+
+// Per-tile image index of a batch: one workgroup per image.
+__global__ __launch_bounds__(256) void k_tile_map(Images ims, Ws ws) {
+    const ImgDev im = ims.table[blockIdx.x];
+    const int64_t n = (int64_t)im.tiles_x * im.tiles_y;
+    for (int64_t t = threadIdx.x; t < n; t += blockDim.x) ws.tile_img[im.tile_begin + t] = blockIdx.x;
 }
 
 // Resets the work queues of the persistent tile stage (graph-safe: a kernel,
@@ -383,7 +390,7 @@ __global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
     bool bad = false;
     const uint32_t cnt = gtile < ws.ntiles ? (ws.counts[gtile] & COUNT_MASK) : 0u;
     if (cnt != 0) {
-        const ImgDev im = locate<MULTI>(ims, gtile);
+        const ImgDev im = locate<MULTI>(ims, ws.tile_img, gtile);
         const int64_t lt = gtile - im.tile_begin;
         const int tx = (int)(lt % im.tiles_x), ty = (int)(lt / im.tiles_x);
         const uint8_t* eq = ws.eq + gtile * 2 * T;
@@ -436,7 +443,7 @@ __global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
 __global__ __launch_bounds__(256) void k_resolve(Ws ws) {
     const uint32_t total = ws.hdr->total;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-        const uint32_t r = gfind(ws.dparent, i);
+        const uint32_t r = gfind_ro(ws.dparent, i);
         if (r != i) g_store(ws.dparent + i, r);
         const uint32_t lb = ws.dbits[i] & 3u;
         if (lb) __hip_atomic_fetch_or(ws.dbits + r, lb << 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -483,7 +490,7 @@ __global__ __launch_bounds__(NT) void k_writeback(Images ims, Ws ws) {
     const uint32_t nd = ws.hdr->dirty_count;
     for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
         const int64_t gtile = ws.dirty[d];
-        const ImgDev im = locate<MULTI>(ims, gtile);
+        const ImgDev im = locate<MULTI>(ims, ws.tile_img, gtile);
         const int64_t lt = gtile - im.tile_begin;
         const int tx = (int)(lt % im.tiles_x), ty = (int)(lt / im.tiles_x);
         const int x0 = tx * T, y0 = ty * T;
@@ -561,6 +568,7 @@ template <bool MULTI>
 static hipError_t launch_all(const Images& ims, const Ws& ws, int T, int NT, bool persistent,
                              int cus, hipStream_t s) {
     const int grid_cap = std::max(1, cus) * 8;
+    if (MULTI) k_tile_map<<<(unsigned)ims.count, 256, 0, s>>>(ims, ws);
     const ProfileHook& hook = profile_hook();
     if (hook.tile_begin) (void)hipEventRecord(hook.tile_begin, s);
 #define CALL_TILE(a, b) launch_tile_kernels<a, b, MULTI>(ims, ws, persistent, grid_cap, s)

@@ -1,0 +1,289 @@
+// ss_harness.cpp -- command-line gate and timing harness for the operator.
+//
+// Mirrors the reference's unified harness (harness/primary/gpu_common_v0_6.cu:
+// CLI :500-535, per-image loop :584-745, timing protocol :305-397, CSV columns
+// :545-553, percentile rule :86-103, byte mask compare :105-121, exit codes
+// :752-755 and :1068-1076 of snapshot_sparse_seam_cuda_v1_0.cu) on top of the
+// C ABI.  Images are binary PGM (P5); OpenCV is not used.
+//
+//   ss_harness <image_dir> --output=DIR [--expected-max=DIR --expected-min=DIR]
+//              [--warmup=5] [--reps=30] [--tile=T --threads=N | --autotune]
+//              [--levels=L] [--device=D] [--skip-expected]
+//
+// Exit codes: 0 all masks match, 1 mismatches, 2 usage, 3 error.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "sparse_seam.hpp"
+
+namespace fs = std::filesystem;
+
+namespace {
+
+struct Pgm {
+    int width = 0, height = 0;
+    std::vector<uint8_t> data;
+};
+
+// Binary PGM reader; the header is tokenised, so a first pixel byte that is
+// itself whitespace is not swallowed.
+Pgm read_pgm(const fs::path& path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("Cannot read input " + path.string());
+    std::vector<char> buf((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    size_t pos = 0;
+    std::vector<std::string> tok;
+    while (tok.size() < 4) {
+        while (pos < buf.size() && std::isspace((unsigned char)buf[pos])) ++pos;
+        if (pos < buf.size() && buf[pos] == '#') {
+            while (pos < buf.size() && buf[pos] != '\n') ++pos;
+            continue;
+        }
+        const size_t s = pos;
+        while (pos < buf.size() && !std::isspace((unsigned char)buf[pos])) ++pos;
+        if (s == pos) throw std::runtime_error("Truncated PGM header " + path.string());
+        tok.emplace_back(buf.data() + s, pos - s);
+    }
+    ++pos;
+    if (tok[0] != "P5" || std::stoi(tok[3]) != 255) throw std::runtime_error("Unsupported PGM " + path.string());
+    Pgm p;
+    p.width = std::stoi(tok[1]);
+    p.height = std::stoi(tok[2]);
+    const size_t n = (size_t)p.width * p.height;
+    if (buf.size() < pos + n) throw std::runtime_error("Truncated PGM data " + path.string());
+    p.data.assign(buf.begin() + pos, buf.begin() + pos + n);
+    return p;
+}
+
+void write_pgm(const fs::path& path, int w, int h, const uint8_t* data) {
+    std::ofstream out(path, std::ios::binary);
+    out << "P5\n" << w << ' ' << h << "\n255\n";
+    out.write(reinterpret_cast<const char*>(data), (std::streamsize)w * h);
+}
+
+// Reference quantiser (harness/quantization_and_scaling/gpu_common_quant_v0_7.cu:479-495).
+void quantize(std::vector<uint8_t>& img, int levels) {
+    if (levels == 256) return;
+    for (auto& v : img) {
+        const int bin = ((int)v * levels) >> 8;
+        v = (uint8_t)((bin * 255 + (levels - 1) / 2) / (levels - 1));
+    }
+}
+
+double percentile(std::vector<double> v, double p) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const double pos = p * (double)(v.size() - 1);
+    const size_t lo = (size_t)std::floor(pos), hi = (size_t)std::ceil(pos);
+    const double f = pos - (double)lo;
+    return v[lo] * (1.0 - f) + v[hi] * f;
+}
+
+uint64_t compare_masks(const std::vector<uint8_t>& a, const std::vector<uint8_t>& b) {
+    if (a.size() != b.size()) throw std::runtime_error("Mask dimensions differ");
+    uint64_t m = 0;
+    for (size_t i = 0; i < a.size(); ++i) m += a[i] != b[i];
+    return m;
+}
+
+std::string quote(const std::string& s) {
+    std::string o = "\"";
+    for (char c : s) {
+        if (c == '"') o.push_back('"');
+        o.push_back(c);
+    }
+    return o + "\"";
+}
+
+bool starts(const std::string& s, const char* p) { return s.rfind(p, 0) == 0; }
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 3) {
+        std::cerr << "Usage: ss_harness <image_dir> --output=DIR [--expected-max=DIR --expected-min=DIR]"
+                     " [--warmup=5] [--reps=30] [--tile=T --threads=N | --autotune] [--levels=L]"
+                     " [--device=D] [--skip-expected]\n";
+        return 2;
+    }
+    try {
+        const fs::path image_dir = argv[1];
+        fs::path out_dir, exp_max, exp_min;
+        int warmup = 5, reps = 30, device = 0, levels = 256;
+        bool skip_expected = false, autotune = false;
+        ss_config cfg{};
+        sparse_seam::check(ss_get_default_config(&cfg), "ss_get_default_config");
+        for (int i = 2; i < argc; ++i) {
+            const std::string a = argv[i];
+            if (starts(a, "--output=")) out_dir = a.substr(9);
+            else if (starts(a, "--expected-max=")) exp_max = a.substr(15);
+            else if (starts(a, "--expected-min=")) exp_min = a.substr(15);
+            else if (starts(a, "--warmup=")) warmup = std::stoi(a.substr(9));
+            else if (starts(a, "--reps=")) reps = std::stoi(a.substr(7));
+            else if (starts(a, "--tile=")) cfg.tile = std::stoi(a.substr(7));
+            else if (starts(a, "--threads=")) cfg.threads = std::stoi(a.substr(10));
+            else if (starts(a, "--levels=")) levels = std::stoi(a.substr(9));
+            else if (starts(a, "--device=")) device = std::stoi(a.substr(9));
+            else if (a == "--skip-expected") skip_expected = true;
+            else if (a == "--autotune") autotune = true;
+            else {
+                std::cerr << "Unknown argument " << a << '\n';
+                return 2;
+            }
+        }
+        if (out_dir.empty()) throw std::runtime_error("Missing --output directory");
+        if (!skip_expected && (exp_max.empty() || exp_min.empty()))
+            throw std::runtime_error("Missing --expected-max/--expected-min (or --skip-expected)");
+        if (levels < 2 || levels > 256) throw std::runtime_error("--levels must be in 2..256");
+        if (reps < 1 || warmup < 0) throw std::runtime_error("--reps must be >= 1 and --warmup >= 0");
+        SS_HIP_CHECK(hipSetDevice(device));
+        hipDeviceProp_t prop{};
+        SS_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+
+        std::vector<fs::path> files;
+        for (const auto& e : fs::directory_iterator(image_dir)) {
+            std::string ext = e.path().extension().string();
+            std::transform(ext.begin(), ext.end(), ext.begin(), ::tolower);
+            if (e.is_regular_file() && ext == ".pgm") files.push_back(e.path());
+        }
+        std::sort(files.begin(), files.end());
+
+        std::vector<Pgm> images;
+        for (const auto& f : files) {
+            images.push_back(read_pgm(f));
+            quantize(images.back().data, levels);
+        }
+
+        if (autotune && !images.empty()) {
+            // time the candidates on (up to) the first 8 images
+            std::vector<sparse_seam::DeviceBuffers> tb(std::min<size_t>(8, images.size()));
+            std::vector<ss_image_desc> d(tb.size());
+            for (size_t i = 0; i < tb.size(); ++i) {
+                sparse_seam::allocate_buffers(tb[i], images[i].width, images[i].height, &cfg);
+                SS_HIP_CHECK(hipMemcpy(tb[i].image, images[i].data.data(), images[i].data.size(),
+                                       hipMemcpyHostToDevice));
+                d[i] = ss_image_desc{tb[i].image, tb[i].maxima, tb[i].minima, (uint64_t)images[i].width,
+                                     (uint64_t)images[i].width, images[i].width, images[i].height};
+            }
+            double rate = 0;
+            sparse_seam::check(ss_autotune(d.data(), (int)d.size(), 5, nullptr, &cfg, &rate), "ss_autotune");
+            for (auto& b : tb) sparse_seam::free_buffers(b);
+            std::cout << "[SS-AUTOTUNE] tile=" << cfg.tile << " threads=" << cfg.threads
+                      << " Mpix_s=" << rate / 1e6 << '\n';
+        }
+
+        fs::create_directories(out_dir / "actual_max");
+        fs::create_directories(out_dir / "actual_min");
+        std::ofstream csv(out_dir / "ss_harness.csv");
+        csv << "algorithm,image,width,height,megapixels,warmup,repetitions,"
+               "block,total_allocated_bytes,total_allocated_Bpp,"
+               "auxiliary_bytes,auxiliary_Bpp,"
+               "median_compute_ms,p95_compute_ms,compute_mpix_s,"
+               "median_h2h_ms,p95_h2h_ms,h2h_mpix_s,"
+               "median_iterations,p95_iterations,"
+               "oracle_max_mismatches,oracle_min_mismatches,"
+               "cross_max_mismatches,cross_min_mismatches,"
+               "tile,active_fragments,dirty_tiles,dsu_retries\n";
+        {
+            std::ofstream env(out_dir / "environment.txt");
+            env << "harness=ss_harness\n"
+                << "device_name=" << prop.name << '\n'
+                << "gcn_arch=" << prop.gcnArchName << '\n'
+                << "compute_units=" << prop.multiProcessorCount << '\n'
+                << "global_memory_bytes=" << prop.totalGlobalMem << '\n'
+                << "tile=" << cfg.tile << '\n'
+                << "threads_per_tile=" << cfg.threads << '\n'
+                << "warmup=" << warmup << '\n'
+                << "repetitions=" << reps << '\n'
+                << "levels=" << levels << '\n'
+                << "timing_compute=preallocated,resident_input,task_complete\n"
+                << "timing_h2h=reused_allocation,H2D+compute+D2H_two_masks\n"
+                << "auxiliary_definition=workspace_bytes(all device buffers except input and two masks)\n";
+        }
+
+        uint64_t total_mismatch = 0;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        SS_HIP_CHECK(hipEventCreate(&e0));
+        SS_HIP_CHECK(hipEventCreate(&e1));
+        for (size_t k = 0; k < files.size(); ++k) {
+            const Pgm& img = images[k];
+            const size_t px = img.data.size();
+            sparse_seam::DeviceBuffers b;
+            sparse_seam::allocate_buffers(b, img.width, img.height, &cfg);
+            std::vector<uint8_t> mx(px), mn(px);
+            SS_HIP_CHECK(hipMemcpy(b.image, img.data.data(), px, hipMemcpyHostToDevice));
+            for (int i = 0; i < warmup; ++i) {
+                sparse_seam::run_sparse_resident(b);
+                SS_HIP_CHECK(hipDeviceSynchronize());
+            }
+            std::vector<double> compute, h2h;
+            for (int i = 0; i < reps; ++i) {
+                SS_HIP_CHECK(hipEventRecord(e0, nullptr));
+                sparse_seam::run_sparse_resident(b);
+                SS_HIP_CHECK(hipEventRecord(e1, nullptr));
+                SS_HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                SS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                compute.push_back(ms);
+            }
+            for (int i = 0; i < reps; ++i) {
+                const auto t0 = std::chrono::high_resolution_clock::now();
+                SS_HIP_CHECK(hipMemcpy(b.image, img.data.data(), px, hipMemcpyHostToDevice));
+                sparse_seam::run_sparse_resident(b);
+                SS_HIP_CHECK(hipMemcpy(mx.data(), b.maxima, px, hipMemcpyDeviceToHost));
+                SS_HIP_CHECK(hipMemcpy(mn.data(), b.minima, px, hipMemcpyDeviceToHost));
+                const auto t1 = std::chrono::high_resolution_clock::now();
+                h2h.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
+            }
+            const ss_stats st = sparse_seam::read_stats(b);
+            uint64_t mm_max = 0, mm_min = 0;
+            const std::string stem = files[k].stem().string();
+            if (!skip_expected) {
+                const Pgm em = read_pgm(exp_max / (stem + ".pgm"));
+                const Pgm en = read_pgm(exp_min / (stem + ".pgm"));
+                mm_max = compare_masks(mx, em.data);
+                mm_min = compare_masks(mn, en.data);
+                write_pgm(out_dir / "actual_max" / (stem + ".pgm"), img.width, img.height, mx.data());
+                write_pgm(out_dir / "actual_min" / (stem + ".pgm"), img.width, img.height, mn.data());
+            }
+            total_mismatch += mm_max + mm_min;
+            const double mp = (double)px / 1e6;
+            const double med_c = percentile(compute, 0.5), med_h = percentile(h2h, 0.5);
+            const uint64_t total_bytes = (uint64_t)px * 3 + b.workspace_bytes;
+            csv << "Sparse-Seam-MI355X," << quote(files[k].filename().string()) << ',' << img.width << ','
+                << img.height << ',' << std::fixed << std::setprecision(6) << mp << ',' << warmup << ','
+                << reps << ',' << cfg.threads << ',' << total_bytes << ',' << (double)total_bytes / px << ','
+                << b.workspace_bytes << ',' << (double)b.workspace_bytes / px << ',' << med_c << ','
+                << percentile(compute, 0.95) << ',' << mp / (med_c * 1e-3) << ',' << med_h << ','
+                << percentile(h2h, 0.95) << ',' << mp / (med_h * 1e-3) << ",0,0," << mm_max << ',' << mm_min
+                << ",0,0," << cfg.tile << ',' << st.fragments << ',' << st.dirty_tiles << ',' << st.dsu_retries
+                << '\n';
+            csv.flush();
+            std::cout << "[SS-HARNESS] " << files[k].filename().string() << ' ' << img.width << 'x'
+                      << img.height << " compute_ms=" << med_c << " h2h_ms=" << med_h
+                      << " fragments=" << st.fragments << " max_mismatch=" << mm_max
+                      << " min_mismatch=" << mm_min << '\n';
+            sparse_seam::free_buffers(b);
+        }
+        SS_HIP_CHECK(hipEventDestroy(e0));
+        SS_HIP_CHECK(hipEventDestroy(e1));
+        std::cout << (skip_expected ? "[SS-HARNESS-MEASUREMENT-OK]" : "[SS-HARNESS-VALIDATION-OK]")
+                  << " images=" << files.size() << " mismatches=" << total_mismatch
+                  << " output=" << out_dir.string() << '\n';
+        return total_mismatch == 0 ? 0 : 1;
+    } catch (const std::exception& e) {
+        std::cerr << "[ERROR] " << e.what() << '\n';
+        return 3;
+    }
+}

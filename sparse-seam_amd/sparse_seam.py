@@ -221,6 +221,106 @@ class BatchPlan:
             pass
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_cap_bytes", ctypes.c_uint64), ("max_batch", ctypes.c_int),
+                ("streams", ctypes.c_int), ("tile", ctypes.c_int), ("threads", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("host_img", ctypes.c_void_p), ("host_max", ctypes.c_void_p),
+                ("host_min", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("tag", ctypes.c_uint64)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("tag", ctypes.c_uint64), ("status", ctypes.c_int32),
+                ("batch_size", ctypes.c_int32), ("submit_s", ctypes.c_double),
+                ("launch_s", ctypes.c_double), ("done_s", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("rejected", ctypes.c_uint64), ("launches", ctypes.c_uint64),
+                ("pixels", ctypes.c_uint64), ("peak_inflight_bytes", ctypes.c_uint64)]
+
+
+def _server_protos(lib):
+    vp = ctypes.c_void_p
+    lib.ss_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+    lib.ss_server_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]
+    lib.ss_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(Completion)]
+    lib.ss_server_drain.argtypes = [vp]
+    lib.ss_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+    lib.ss_server_destroy.argtypes = [vp]
+    lib.ss_server_run_open_loop.argtypes = [vp, ctypes.POINTER(Request), ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_uint64, ctypes.POINTER(Completion)]
+    for n in ("ss_server_create", "ss_server_submit", "ss_server_wait", "ss_server_drain",
+              "ss_server_get_stats", "ss_server_destroy", "ss_server_run_open_loop"):
+        getattr(lib, n).restype = ctypes.c_int
+
+
+class Server:
+    """Continuous-batching request loop (ss_server_*).  Requests are host
+    numpy arrays; outputs are written into host arrays owned by the caller."""
+
+    def __init__(self, hbm_cap_bytes, max_batch=0, streams=2, tile=0, threads=0):
+        lib = _lib()
+        if not hasattr(lib, "_server_ready"):
+            _server_protos(lib)
+            lib._server_ready = True
+        self._h = ctypes.c_void_p()
+        cfg = ServerConfig(hbm_cap_bytes, max_batch, streams, tile, threads)
+        _check(lib.ss_server_create(ctypes.byref(cfg), ctypes.byref(self._h)), "ss_server_create")
+        self._keep = {}
+
+    @staticmethod
+    def request(img, mx, mn, tag=0):
+        h, w = img.shape
+        for a in (img, mx, mn):
+            assert a.flags["C_CONTIGUOUS"] and a.shape == (h, w)
+        return Request(img.ctypes.data, mx.ctypes.data, mn.ctypes.data, w, h, tag)
+
+    def submit(self, img, mx, mn, tag=0):
+        rid = ctypes.c_uint64()
+        req = self.request(img, mx, mn, tag)
+        _check(_lib().ss_server_submit(self._h, ctypes.byref(req), ctypes.byref(rid)), "ss_server_submit")
+        self._keep[rid.value] = (img, mx, mn)
+        return rid.value
+
+    def wait(self, rid):
+        c = Completion()
+        st = _lib().ss_server_wait(self._h, rid, ctypes.byref(c))
+        self._keep.pop(rid, None)
+        _check(st, "ss_server_wait")
+        return c
+
+    def drain(self):
+        _check(_lib().ss_server_drain(self._h), "ss_server_drain")
+
+    def stats(self):
+        s = ServerStats()
+        _check(_lib().ss_server_get_stats(self._h, ctypes.byref(s)), "ss_server_get_stats")
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def run_open_loop(self, reqs, qps, seed=0):
+        """reqs: ctypes array of Request.  Returns a ctypes array of Completion."""
+        out = (Completion * len(reqs))()
+        _check(_lib().ss_server_run_open_loop(self._h, reqs, len(reqs), qps, seed, out),
+               "ss_server_run_open_loop")
+        return out
+
+    def close(self):
+        if self._h:
+            _lib().ss_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def profile_events(begin=None, end=None):
     """Route the tile-stage kernel through a pair of torch.cuda.Event objects
     (recorded on the pipeline's own stream); None, None turns it off."""
