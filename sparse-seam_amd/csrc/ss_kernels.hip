@@ -12,192 +12,9 @@
 // (usually few) tiles whose seam zones change.
 #include "ss_device.hpp"
 #include "ss_internal.hpp"
+#include "ss_tile.hpp"
 
 namespace ssx {
-
-template <int T>
-struct TileSmem {
-    uint8_t img[(T + 2) * (T + 2)];   // tile + one-pixel halo (replicated at image borders)
-    uint8_t pflag[T * T];             // per-pixel G/S/SEAM evidence (write-back: root -> fragment map)
-    uint32_t lab[T * T];              // labels; at roots also zone evidence and fragment id
-    uint8_t eqr[T];
-    uint8_t eqb[T];
-    uint32_t count;
-    uint32_t err;
-    uint32_t next;                    // work-queue broadcast
-};
-
-// Loads the tile (and, with HALO, its one-pixel halo) into LDS.  Halo pixels
-// outside the image replicate the nearest image pixel: such a value always
-// belongs to the pixel itself or to one of its genuine 8-neighbours, so the
-// 3x3 greater / smaller tests equal the "absent neighbour" rule exactly.
-template <int T, int NT, bool HALO>
-__device__ __forceinline__ void tile_load(TileSmem<T>& sm, const ImgDev& im, int x0, int y0,
-                                          int tw, int th) {
-    constexpr int P = T + 2;
-    if (HALO) {
-        for (int i = threadIdx.x; i < P * P; i += NT) {
-            const int ly = i / P - 1, lx = i % P - 1;
-            if (ly > th || lx > tw) continue;
-            int gy = y0 + ly, gx = x0 + lx;
-            gy = gy < 0 ? 0 : (gy >= im.h ? im.h - 1 : gy);
-            gx = gx < 0 ? 0 : (gx >= im.w ? im.w - 1 : gx);
-            sm.img[i] = im.img[(int64_t)gy * im.pitch + gx];
-        }
-    } else {
-        for (int p = threadIdx.x; p < T * T; p += NT) {
-            const int lx = p % T, ly = p / T;
-            if (lx >= tw || ly >= th) continue;
-            sm.img[(ly + 1) * P + lx + 1] = im.img[(int64_t)(y0 + ly) * im.pitch + x0 + lx];
-        }
-    }
-}
-
-// Komura-equivalence labelling of the tile's flat zones (8-connectivity):
-// link every pixel to its first equal predecessor (up, up-left, up-right,
-// left), compress, unite the left and up-right edges, compress.  Every link
-// points to a smaller index, so each zone's root is its minimum raster index.
-// With CLASSIFY the first pass also records the pixel's evidence bits and the
-// cross-seam equality bits of the right column and bottom row.
-template <int T, int NT, bool CLASSIFY>
-__device__ __forceinline__ void tile_label(TileSmem<T>& sm, const ImgDev& im, int x0, int y0,
-                                           int tw, int th) {
-    constexpr int P = T + 2;
-    const bool hasL = x0 > 0, hasR = x0 + tw < im.w, hasU = y0 > 0, hasD = y0 + th < im.h;
-    for (int p = threadIdx.x; p < T * T; p += NT) {
-        const int lx = p % T, ly = p / T;
-        if (lx >= tw || ly >= th) continue;
-        const uint8_t* c = &sm.img[(ly + 1) * P + lx + 1];
-        const uint8_t v = c[0];
-        const uint8_t nu = c[-P], nul = c[-P - 1], nur = c[-P + 1], nl = c[-1];
-        uint32_t parent = p;
-        if (ly > 0 && nu == v) parent = p - T;
-        else if (ly > 0 && lx > 0 && nul == v) parent = p - T - 1;
-        else if (ly > 0 && lx + 1 < tw && nur == v) parent = p - T + 1;
-        else if (lx > 0 && nl == v) parent = p - 1;
-        sm.lab[p] = parent;
-        if (CLASSIFY) {
-            const uint8_t nr = c[1], ndl = c[P - 1], nd = c[P], ndr = c[P + 1];
-            const uint8_t hi = max(max(max(nu, nul), max(nur, nl)), max(max(nr, ndl), max(nd, ndr)));
-            const uint8_t lo = min(min(min(nu, nul), min(nur, nl)), min(min(nr, ndl), min(nd, ndr)));
-            uint32_t f = (hi > v ? F_G : 0u) | (lo < v ? F_S : 0u);
-            const bool perim = lx == 0 || ly == 0 || lx == tw - 1 || ly == th - 1;
-            if (perim) {
-                // equal neighbour outside the tile but inside the image
-                auto seam = [&](int dx, int dy, uint8_t n) -> uint32_t {
-                    const int nx = lx + dx, ny = ly + dy;
-                    const bool out = nx < 0 || nx >= tw || ny < 0 || ny >= th;
-                    const bool in_img = (nx >= 0 || hasL) && (nx < tw || hasR) &&
-                                        (ny >= 0 || hasU) && (ny < th || hasD);
-                    return (out && in_img && n == v) ? 1u : 0u;
-                };
-                const uint32_t sul = seam(-1, -1, nul), su = seam(0, -1, nu), sur = seam(1, -1, nur);
-                const uint32_t sl = seam(-1, 0, nl), sr = seam(1, 0, nr);
-                const uint32_t sdl = seam(-1, 1, ndl), sd = seam(0, 1, nd), sdr = seam(1, 1, ndr);
-                if (sul | su | sur | sl | sr | sdl | sd | sdr) f |= F_SEAM;
-                if (lx == tw - 1) sm.eqr[ly] = (uint8_t)(sur | (sr << 1) | (sdr << 2));
-                if (ly == th - 1) sm.eqb[lx] = (uint8_t)(sdl | (sd << 1) | (sdr << 2));
-            }
-            sm.pflag[p] = (uint8_t)f;
-        }
-    }
-    __syncthreads();
-    for (int p = threadIdx.x; p < T * T; p += NT) {
-        if (p % T < tw && p / T < th) sm.lab[p] = lfind(sm.lab, p);
-    }
-    __syncthreads();
-    for (int p = threadIdx.x; p < T * T; p += NT) {
-        const int lx = p % T, ly = p / T;
-        if (lx >= tw || ly >= th) continue;
-        const uint8_t* c = &sm.img[(ly + 1) * P + lx + 1];
-        const uint8_t v = c[0];
-        if (lx > 0 && c[-1] == v) lunion(sm.lab, p, p - 1);
-        if (ly > 0 && lx + 1 < tw && c[-P + 1] == v) lunion(sm.lab, p, p - T + 1);
-    }
-    __syncthreads();
-    for (int p = threadIdx.x; p < T * T; p += NT) {
-        if (p % T < tw && p / T < th) sm.lab[p] = lfind(sm.lab, p);
-    }
-    __syncthreads();
-}
-
-// Tile stage for one tile (see ss_device.hpp, step 1).
-template <int T, int NT>
-__device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, int64_t gtile,
-                                           const Ws& ws) {
-    constexpr int S = 4 * T - 4;
-    const int64_t lt = gtile - im.tile_begin;
-    const int tx = (int)(lt % im.tiles_x), ty = (int)(lt / im.tiles_x);
-    const int x0 = tx * T, y0 = ty * T;
-    const int tw = min(T, im.w - x0), th = min(T, im.h - y0);
-    if (threadIdx.x == 0) {
-        sm.count = 0;
-        sm.err = 0;
-    }
-    tile_load<T, NT, true>(sm, im, x0, y0, tw, th);
-    __syncthreads();
-    tile_label<T, NT, true>(sm, im, x0, y0, tw, th);
-
-    // OR the pixel evidence into the zone root (skipped when there is none).
-    for (int p = threadIdx.x; p < T * T; p += NT) {
-        if (p % T >= tw || p / T >= th) continue;
-        const uint32_t f = sm.pflag[p];
-        if (f) __hip_atomic_fetch_or(&sm.lab[sm.lab[p] & LMASK], f << FLAG_SHIFT, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-    __syncthreads();
-
-    // Number the seam zones and fill their fragment slots.
-    for (int p = threadIdx.x; p < T * T; p += NT) {
-        if (p % T >= tw || p / T >= th) continue;
-        const uint32_t w = sm.lab[p];
-        if ((w & LMASK) != (uint32_t)p || !((w >> FLAG_SHIFT) & F_SEAM)) continue;
-        const uint32_t j = atomicAdd(&sm.count, 1u);
-        if (j >= (uint32_t)S) {
-            sm.err = 1u;
-            continue;
-        }
-        sm.lab[p] = w | ((j + 1) << FRAG_SHIFT);
-        ws.sbits[gtile * S + j] = (uint8_t)((w >> FLAG_SHIFT) & 3u);
-        ws.sroot[gtile * S + j] = (uint16_t)p;
-    }
-    __syncthreads();
-
-    // Masks (final for tile-contained zones, provisional for seam zones) and
-    // the perimeter records of seam pixels.
-    uint8_t* rec = ws.rec + gtile * 4 * T;
-    for (int p = threadIdx.x; p < T * T; p += NT) {
-        const int lx = p % T, ly = p / T;
-        if (lx >= tw || ly >= th) continue;
-        const uint32_t rw = sm.lab[sm.lab[p] & LMASK];
-        const uint32_t zf = rw >> FLAG_SHIFT;
-        const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx;
-        im.mx[o] = (zf & F_G) ? 0 : 255;
-        im.mn[o] = (zf & F_S) ? 0 : 255;
-        if (sm.pflag[p] & F_SEAM) {
-            const uint32_t jr = (rw >> FRAG_SHIFT) & 0xFFu;
-            if (jr == 0u) {
-                sm.err = 2u;
-                continue;
-            }
-            const uint8_t j = (uint8_t)(jr - 1u);
-            if (ly == 0) rec[lx] = j;
-            if (ly == th - 1) rec[T + lx] = j;
-            if (lx == 0) rec[2 * T + ly] = j;
-            if (lx == tw - 1) rec[3 * T + ly] = j;
-        }
-    }
-    __syncthreads();
-    const uint32_t count = sm.count;
-    if (count) {
-        uint8_t* eq = ws.eq + gtile * 2 * T;
-        for (int k = threadIdx.x; k < T; k += NT) {
-            eq[k] = (k < th && tw == T) ? sm.eqr[k] : 0;
-            eq[T + k] = (k < tw && th == T) ? sm.eqb[k] : 0;
-        }
-    }
-    if (threadIdx.x == 0) ws.counts[gtile] = min(count, (uint32_t)S) | (sm.err << COUNT_ERR_SHIFT);
-}
 
 // Plain form: one workgroup per tile, XCD-aware tile order.
 template <int T, int NT, bool MULTI>
@@ -217,16 +34,19 @@ template <int T, int NT, bool MULTI>
 __global__ __launch_bounds__(NT) void k_tile_stage_persistent(Images ims, Ws ws, int chunk) {
     __shared__ TileSmem<T> sm;
     const int home = blockIdx.x % NUM_XCD;
+#pragma unroll 1
     for (int q = 0; q < NUM_XCD; ++q) {
         const int xq = (home + q) % NUM_XCD;
         for (;;) {
             __syncthreads();
             if (threadIdx.x == 0) sm.next = atomicAdd(&ws.hdr->queue[xq], (uint32_t)chunk);
             __syncthreads();
-            const uint32_t start = sm.next;
+            // values read from LDS are not known to be uniform; make them scalar
+            const uint32_t start = __builtin_amdgcn_readfirstlane(sm.next);
             const uint32_t end_q = ws.hdr->queue_end[xq];
             if (start >= end_q) break;
             const uint32_t stop = min(start + (uint32_t)chunk, end_q);
+#pragma unroll 1
             for (uint32_t t = start; t < stop; ++t) {
                 const ImgDev im = locate<MULTI>(ims, ws.tile_img, t);
                 tile_stage<T, NT>(sm, im, t, ws);
@@ -486,36 +306,12 @@ __global__ __launch_bounds__(256) void k_mark(Ws ws) {
 template <int T, int NT, bool MULTI>
 __global__ __launch_bounds__(NT) void k_writeback(Images ims, Ws ws) {
     __shared__ TileSmem<T> sm;
-    constexpr int S = 4 * T - 4;
     const uint32_t nd = ws.hdr->dirty_count;
+#pragma unroll 1
     for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
         const int64_t gtile = ws.dirty[d];
         const ImgDev im = locate<MULTI>(ims, ws.tile_img, gtile);
-        const int64_t lt = gtile - im.tile_begin;
-        const int tx = (int)(lt % im.tiles_x), ty = (int)(lt / im.tiles_x);
-        const int x0 = tx * T, y0 = ty * T;
-        const int tw = min(T, im.w - x0), th = min(T, im.h - y0);
-        for (int p = threadIdx.x; p < T * T; p += NT) sm.pflag[p] = 0;
-        tile_load<T, NT, false>(sm, im, x0, y0, tw, th);
-        __syncthreads();
-        tile_label<T, NT, false>(sm, im, x0, y0, tw, th);
-        const uint32_t cnt = ws.counts[gtile] & COUNT_MASK;
-        const uint32_t b = ws.base[gtile];
-        for (uint32_t j = threadIdx.x; j < cnt; j += NT) {
-            if (ws.dbits[b + j] & DB_CHANGED) sm.pflag[ws.sroot[gtile * S + j]] = (uint8_t)(j + 1);
-        }
-        __syncthreads();
-        for (int p = threadIdx.x; p < T * T; p += NT) {
-            const int lx = p % T, ly = p / T;
-            if (lx >= tw || ly >= th) continue;
-            const uint32_t j1 = sm.pflag[sm.lab[p]];
-            if (!j1) continue;
-            const uint32_t res = (ws.dbits[ws.dparent[b + j1 - 1]] >> 2) | ws.dbits[b + j1 - 1];
-            const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx;
-            im.mx[o] = (res & F_G) ? 0 : 255;
-            im.mn[o] = (res & F_S) ? 0 : 255;
-        }
-        __syncthreads();
+        tile_writeback<T, NT>(sm, im, gtile, ws);
     }
 }
 
