@@ -125,6 +125,12 @@ def main():
     img_ev = [[(ev(), ev()) for _ in range(n)] for _ in range(K)]
     k1_ev = [[(ev(), ev()) for _ in range(n)] for _ in range(K)]
     start, stop = ev(), ev()
+    # torch creates the HIP event lazily at the first record; record the
+    # tile-stage events once so their handles exist before the library uses them
+    for row in k1_ev:
+        for ka, kb in row:
+            ka.record(stream)
+            kb.record(stream)
     torch.cuda.synchronize()
     start.record(stream)
     for k in range(K):

@@ -13,7 +13,7 @@
 namespace ssx {
 
 static std::mutex g_cfg_mu;
-static ss_config g_default_cfg = {32, 256};
+static ss_config g_default_cfg = {64, 256};
 
 ss_config resolve_config(const ss_config* cfg) {
     if (cfg) return *cfg;
@@ -237,13 +237,23 @@ ss_status ss_batch_plan_create(const ss_image_desc* images, int n, void* d_ws, s
     }
     ss_batch_plan* p = new (std::nothrow) ss_batch_plan();
     if (!p) return SS_ERR_CAPACITY;
-    if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)device_cus();  // cache the device query outside the capture
+    // Capture on a private stream: the caller's stream may be the null stream,
+    // which cannot be captured.  The graph is later launched on any stream.
+    hipStream_t cap = nullptr;
+    if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
         delete p;
         return SS_ERR_HIP;
     }
-    st = batch_enqueue(images, n, d_ws, ws_bytes, c, stream, false);
+    if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipStreamDestroy(cap);
+        delete p;
+        return SS_ERR_HIP;
+    }
+    st = batch_enqueue(images, n, d_ws, ws_bytes, c, cap, false);
     hipGraph_t g = nullptr;
-    const hipError_t ec = hipStreamEndCapture(stream, &g);
+    const hipError_t ec = hipStreamEndCapture(cap, &g);
+    (void)hipStreamDestroy(cap);
     if (st != SS_OK || ec != hipSuccess || !g) {
         if (g) (void)hipGraphDestroy(g);
         delete p;

@@ -18,9 +18,10 @@
 //   4. resolve      full compression and OR of the local evidence into roots.
 //   5. mark         fragments whose resolved evidence differs from the local
 //                   evidence are flagged; their tiles go to a dirty-tile list.
-//   6. write-back   dirty tiles are relabelled in LDS (the labels are the same
-//                   deterministic min-index roots as in step 1) and only the
-//                   pixels of changed fragments are rewritten.
+//   6. write-back   dirty tiles stream their 1-byte per-pixel record plane
+//                   (written in step 1 only by tiles holding a fragment that
+//                   lacks one kind of evidence) and rewrite only the pixels of
+//                   changed fragments.
 //
 // Everything is exact integer work; no MFMA (nothing here is a contraction).
 #pragma once
@@ -76,11 +77,14 @@ struct WsHeader {
 };
 static_assert(sizeof(WsHeader) == 256, "header must be 256 bytes");
 
-// counts[t] holds the tile's seam-fragment count in bits 0-27 and the
-// tile-stage invariant code in bits 28-31 (folded into WsHeader::error by the
-// scan, so no per-run clearing is needed before the tile stage).
-constexpr uint32_t COUNT_MASK = 0x0FFFFFFFu;
+// counts[t] holds the tile's seam-fragment count in bits 0-23, two tile flags,
+// and the tile-stage invariant code in bits 28-31 (folded into
+// WsHeader::error by the scan, so no per-run clearing is needed).
+constexpr uint32_t COUNT_MASK = 0x00FFFFFFu;
+constexpr uint32_t CNT_UNIFORM = 1u << 24;   // the tile is one zone (fragment 0 if seam)
+constexpr uint32_t CNT_OPEN = 1u << 25;      // the tile wrote its per-pixel record plane
 constexpr int COUNT_ERR_SHIFT = 28;
+constexpr uint8_t REC_NONE = 255;            // record of a pixel outside every open fragment
 
 struct Ws {
     WsHeader* hdr;
@@ -89,7 +93,8 @@ struct Ws {
     uint32_t* bsum;      // [nblocks + 1] scan block sums (last = total)
     uint32_t* berr;      // [nblocks] OR of the tile invariant codes per scan block
     uint8_t* sbits;      // [tiles * S] local G/S evidence per fragment slot
-    uint16_t* sroot;     // [tiles * S] tile-local root index per fragment slot
+    uint8_t* rec8;       // [tiles * T*T] per-pixel record (open tiles only): the
+                         //   pixel's open fragment, or REC_NONE
     uint8_t* rec;        // [tiles * 4T] fragment id of seam perimeter pixels
                          //   [0,T) top row, [T,2T) bottom row, [2T,3T) left col, [3T,4T) right col
     uint8_t* eq;         // [tiles * 2T] cross-seam equality bits: [0,T) right col, [T,2T) bottom row
@@ -158,37 +163,34 @@ __device__ __forceinline__ void lunion(uint32_t* lab, uint32_t a, uint32_t b) {
     }
 }
 
-// ---- global union-find over dense fragment ids (agent scope) -------------
+// ---- global union-find over dense fragment ids ----------------------------
+//
+// Walks use ordinary (cached) loads and plain halving stores.  A cached parent
+// may be stale, but it is always an ancestor that was current at some point:
+// a link only ever turns a root into a child of a smaller id, and halving only
+// rewrites a non-root (which never becomes a root again) to one of its
+// ancestors.  Links are agent-scope CAS on a node believed to be a root.  A
+// failed CAS returns that node's current parent, a strictly smaller id, and
+// the next walk resumes from it, so every retry lowers max(a, b) and the
+// union terminates.
 
-__device__ __forceinline__ uint32_t g_load(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void g_store(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Find with path halving.  Only non-roots are rewritten, and always to an
-// ancestor, so the stores never race with a link (links CAS roots only).
 __device__ __forceinline__ uint32_t gfind(uint32_t* parent, uint32_t x) {
     for (;;) {
-        const uint32_t p = g_load(parent + x);
+        const uint32_t p = parent[x];
         if (p == x) return x;
-        const uint32_t gp = g_load(parent + p);
+        const uint32_t gp = parent[p];
         if (gp == p) return p;
-        g_store(parent + x, gp);
+        parent[x] = gp;
         x = gp;
     }
 }
 
-// Read-only find, for full compression after all unions: every thread then
-// writes only its own parent entry (with the root), so no halving store of
-// another thread can overwrite a root with a mere ancestor.
+// Read-only walk to the root (for consumers after all links are done).
 __device__ __forceinline__ uint32_t gfind_ro(const uint32_t* parent, uint32_t x) {
-    uint32_t r = g_load(parent + x);
     for (;;) {
-        const uint32_t n = g_load(parent + r);
-        if (n == r) return r;
-        r = n;
+        const uint32_t p = parent[x];
+        if (p == x) return x;
+        x = p;
     }
 }
 
@@ -204,7 +206,7 @@ __device__ __forceinline__ void gunion(uint32_t* parent, uint32_t a, uint32_t b,
                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             return;
         ++retries;
-        a = hi;
+        a = expected;
         b = lo;
     }
 }

@@ -7,9 +7,10 @@
 //   seam merge (3 pairs per seam px):508-581  -> k_seam_merge
 //   compress + reduce               :583-619  -> k_resolve
 //   materialise seam records        :477-506  -> k_mark + k_writeback
-// The record-per-pixel write-back of the reference (1 B written and re-read for
-// every pixel) is replaced by provisional masks plus an LDS relabel of the
-// (usually few) tiles whose seam zones change.
+// The reference writes a 1-byte record for every pixel and re-reads it for
+// every pixel; here masks are written once (provisionally for seam zones), the
+// record plane is written only by tiles holding a fragment that can still
+// change, and only the tiles whose fragments did change are streamed again.
 #include "ss_device.hpp"
 #include "ss_internal.hpp"
 #include "ss_tile.hpp"
@@ -201,13 +202,20 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(Ws ws) {
 // One thread per (tile, k): the tile's right-column pixel k against the three
 // pixels right of it, and its bottom-row pixel k against the three below it.
 // Together these visit every equal pixel pair that straddles a tile seam.
+// A plateau crossing a seam yields the same fragment pair at many consecutive
+// k, so a pair already present in the thread or in the neighbouring lane (the
+// previous k of the same tile edge) is not united again.
 template <int T, bool MULTI>
 __global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t gtile = g / T;
     const int k = (int)(g % T);
+    const int lane = threadIdx.x & 63;
     uint32_t retries = 0;
     bool bad = false;
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    uint32_t ra = NONE, rb[3] = {NONE, NONE, NONE};   // right edge: fa and up to 3 fb
+    uint32_t ba = NONE, bb[3] = {NONE, NONE, NONE};   // bottom edge
     const uint32_t cnt = gtile < ws.ntiles ? (ws.counts[gtile] & COUNT_MASK) : 0u;
     if (cnt != 0) {
         const ImgDev im = locate<MULTI>(ims, ws.tile_img, gtile);
@@ -220,16 +228,17 @@ __global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
             const uint32_t e = eq[k];
             if (e) {
                 const uint32_t ja = rec[3 * T + k];
-                const uint32_t fa = base + ja;
+                if (ja < cnt) ra = base + ja; else bad = true;
                 const int y = ty * T + k;
+#pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    if (!(e & (1u << d))) continue;
+                    if (!(e & (1u << d)) || ra == NONE) continue;
                     const int ny = y + d - 1;
                     const int nty = ny / T;
                     const int64_t nt = im.tile_begin + (int64_t)nty * im.tiles_x + tx + 1;
                     const uint32_t jb = ws.rec[nt * 4 * T + 2 * T + (ny - nty * T)];
-                    if (ja >= cnt || jb >= (ws.counts[nt] & COUNT_MASK)) { bad = true; continue; }
-                    gunion(ws.dparent, fa, ws.base[nt] + jb, retries);
+                    if (jb >= (ws.counts[nt] & COUNT_MASK)) { bad = true; continue; }
+                    rb[d] = ws.base[nt] + jb;
                 }
             }
         }
@@ -237,22 +246,45 @@ __global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
             const uint32_t e = eq[T + k];
             if (e) {
                 const uint32_t ja = rec[T + k];
-                const uint32_t fa = base + ja;
+                if (ja < cnt) ba = base + ja; else bad = true;
                 const int x = tx * T + k;
+#pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    if (!(e & (1u << d))) continue;
+                    if (!(e & (1u << d)) || ba == NONE) continue;
                     const int nx = x + d - 1;
                     const int ntx = nx / T;
                     const int64_t nt = im.tile_begin + (int64_t)(ty + 1) * im.tiles_x + ntx;
                     const uint32_t jb = ws.rec[nt * 4 * T + (nx - ntx * T)];
-                    if (ja >= cnt || jb >= (ws.counts[nt] & COUNT_MASK)) { bad = true; continue; }
-                    gunion(ws.dparent, fa, ws.base[nt] + jb, retries);
+                    if (jb >= (ws.counts[nt] & COUNT_MASK)) { bad = true; continue; }
+                    bb[d] = ws.base[nt] + jb;
                 }
             }
         }
     }
+    // pairs of the previous k (lane - 1) on the same kind of edge
+    const uint32_t pra = __shfl_up(ra, 1, 64), pba = __shfl_up(ba, 1, 64);
+    uint32_t prb[3], pbb[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        prb[d] = __shfl_up(rb[d], 1, 64);
+        pbb[d] = __shfl_up(bb[d], 1, 64);
+    }
+    const bool has_prev = lane > 0 && k > 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (rb[d] != NONE) {
+            bool dup = (d > 0 && rb[d] == rb[d - 1]) || (d > 1 && rb[d] == rb[d - 2]);
+            if (has_prev && pra == ra) dup = dup || rb[d] == prb[0] || rb[d] == prb[1] || rb[d] == prb[2];
+            if (!dup) gunion(ws.dparent, ra, rb[d], retries);
+        }
+        if (bb[d] != NONE) {
+            bool dup = (d > 0 && bb[d] == bb[d - 1]) || (d > 1 && bb[d] == bb[d - 2]);
+            if (has_prev && pba == ba) dup = dup || bb[d] == pbb[0] || bb[d] == pbb[1] || bb[d] == pbb[2];
+            if (!dup) gunion(ws.dparent, ba, bb[d], retries);
+        }
+    }
     const uint32_t r = wave_sum(retries);
-    if ((threadIdx.x & 63) == 0 && r) atomicAdd((unsigned long long*)&ws.hdr->retries, (unsigned long long)r);
+    if (lane == 0 && r) atomicAdd((unsigned long long*)&ws.hdr->retries, (unsigned long long)r);
     // a perimeter record out of range would mean a broken tile stage: report
     // it instead of following it
     if (bad) atomicOr(&ws.hdr->error, 2u);
@@ -260,13 +292,38 @@ __global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
 
 // ---- resolve: compress + OR local evidence into the roots ----------------
 
+// A component can hold millions of fragments (a near-percolating plateau), so
+// OR-ing every fragment's evidence into the one root word would serialise on
+// that address.  Each wave therefore issues one atomic per distinct root and
+// only for bits the (possibly stale, never wrong) cached root word lacks.
 __global__ __launch_bounds__(256) void k_resolve(Ws ws) {
     const uint32_t total = ws.hdr->total;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-        const uint32_t r = gfind_ro(ws.dparent, i);
-        if (r != i) g_store(ws.dparent + i, r);
-        const uint32_t lb = ws.dbits[i] & 3u;
-        if (lb) __hip_atomic_fetch_or(ws.dbits + r, lb << 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int lane = threadIdx.x & 63;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t wbase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < total; wbase += stride) {
+        const uint32_t i = wbase + lane;
+        uint32_t r = 0, lb = 0;
+        if (i < total) {
+            // halving walk: compresses the trees for the other threads; a later
+            // halving store may replace this entry with an ancestor, so
+            // consumers finish the walk with gfind_ro
+            r = gfind(ws.dparent, i);
+            if (r != i) ws.dparent[i] = r;
+            lb = ws.dbits[i] & 3u;
+            if (lb && ((ws.dbits[r] >> 2) & lb) == lb) lb = 0;
+        }
+        uint64_t pending = __ballot(lb != 0);
+        while (pending) {
+            const int leader = __ffsll((unsigned long long)pending) - 1;
+            const uint32_t rl = __shfl(r, leader, 64);
+            const bool mine = lb != 0 && r == rl;
+            const uint64_t m = __ballot(mine);
+            const uint32_t bits = (__ballot(mine && (lb & 1u)) ? 1u : 0u) | (__ballot(mine && (lb & 2u)) ? 2u : 0u);
+            if (lane == leader)
+                __hip_atomic_fetch_or(ws.dbits + rl, bits << 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (mine) lb = 0;
+            pending &= ~m;
+        }
     }
 }
 
@@ -281,7 +338,7 @@ __global__ __launch_bounds__(256) void k_mark(Ws ws) {
         for (uint32_t j = 0; j < cnt; ++j) {
             const uint32_t i = b + j;
             const uint32_t own = ws.dbits[i];
-            const uint32_t res = (ws.dbits[ws.dparent[i]] >> 2) & 3u;
+            const uint32_t res = (ws.dbits[gfind_ro(ws.dparent, i)] >> 2) & 3u;
             if (res & ~own & 3u) {
                 ws.dbits[i] = own | DB_CHANGED;
                 dirty = true;

@@ -12,8 +12,8 @@
 //   singleton pixels  no equal 8-neighbour at all: the zone is the pixel, its
 //                     evidence is final, and it takes no label or atomic;
 //   uniform tiles     every pixel has one value: one zone, no labelling.
-// Both produce the same labels (minimum raster index per zone) as the general
-// path, so the write-back relabel and the fragment roots agree.
+// Both produce the same zones (roots are minimum raster indices) as the
+// general path.
 #pragma once
 #include "ss_device.hpp"
 
@@ -31,13 +31,14 @@ struct TileSmem {
     static constexpr int P = T + 8;   // LDS row pitch in bytes
     alignas(16) uint8_t img[(T + 2) * P];
     alignas(16) uint32_t lab[T * T];  // labels; at roots also evidence and fragment id
-    alignas(16) uint8_t map[T * T];   // write-back: root -> changed fragment + 1
+    uint8_t lut[4 * T];               // write-back: fragment -> 0x80 | new mask bits, or 0
     uint8_t eqr[T];                   // cross-seam equality bits, right column
     uint8_t eqb[T];                   // cross-seam equality bits, bottom row
     uint32_t count;
     uint32_t err;
     uint32_t next;                    // work-queue broadcast
     uint32_t zflags;                  // uniform tile: OR of the evidence
+    uint32_t open;                    // the tile holds a fragment that may still change
 };
 
 template <int T, int NT>
@@ -260,6 +261,7 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
         sm.count = 0;
         sm.err = 0;
         sm.zflags = 0;
+        sm.open = 0;
     }
     tile_load<T, NT, true>(sm, im, x0, y0, tw, th);
     __syncthreads();
@@ -283,7 +285,6 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
         if (threadIdx.x == 0 && (zf & F_SEAM)) {
             sm.count = 1;
             ws.sbits[gtile * S] = (uint8_t)(zf & 3u);
-            ws.sroot[gtile * S] = 0;
         }
         const uint32_t mxb = (zf & F_G) ? 0u : 0xFFFFFFFFu, mnb = (zf & F_S) ? 0u : 0xFFFFFFFFu;
 #pragma unroll
@@ -323,18 +324,22 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
                 }
                 sm.lab[p] = w | ((j + 1) << FRAG_SHIFT);
                 ws.sbits[gtile * S + j] = (uint8_t)((w >> FLAG_SHIFT) & 3u);
-                ws.sroot[gtile * S + j] = (uint16_t)p;
+                // a fragment that already has both kinds of evidence is final
+                if (((w >> FLAG_SHIFT) & 3u) != 3u) sm.open = 1u;
             }
         }
         __syncthreads();
-        // Masks (final for tile-contained zones, provisional for seam zones)
-        // and the perimeter records of seam pixels.
+        // Masks (final for tile-contained zones, provisional for seam zones),
+        // the perimeter records of seam pixels and, when the tile holds an open
+        // fragment, its per-pixel record plane for the write-back.
+        const bool open = sm.open != 0;
+        uint8_t* plane = ws.rec8 + gtile * (int64_t)(T * T);
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const uint32_t sel = (st[k] >> Q_VAL) & 0xFu;
             if (!sel) continue;
             const int q = threadIdx.x + k * NT, ly = q / D, lx0 = 4 * (q % D);
-            uint32_t mxw = 0, mnw = 0;
+            uint32_t mxw = 0, mnw = 0, rcw = 0xFFFFFFFFu;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 if (!((sel >> i) & 1u)) continue;
@@ -345,6 +350,8 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
                 } else {
                     rw = sm.lab[sm.lab[p] & LMASK];
                     zf = rw >> FLAG_SHIFT;
+                    const uint32_t jr = (rw >> FRAG_SHIFT) & 0xFFu;
+                    if (jr && (zf & 3u) != 3u) rcw ^= (0xFFu ^ (jr - 1u)) << (8 * i);
                 }
                 mxw |= ((zf & F_G) ? 0u : 0xFFu) << (8 * i);
                 mnw |= ((zf & F_S) ? 0u : 0xFFu) << (8 * i);
@@ -364,10 +371,12 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
             const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
             store_quad(im.mx + o, mxw, sel, al_out);
             store_quad(im.mn + o, mnw, sel, al_out);
+            if (open) *reinterpret_cast<uint32_t*>(plane + ly * T + lx0) = rcw;
         }
     }
     __syncthreads();
     const uint32_t count = sm.count;
+    const uint32_t tflags = (uniform ? CNT_UNIFORM : 0u) | (sm.open ? CNT_OPEN : 0u);
     if (count) {
         uint8_t* eq = ws.eq + gtile * 2 * T;
         for (int k = threadIdx.x; k < T; k += NT) {
@@ -375,49 +384,59 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
             eq[T + k] = (k < tw && th == T) ? sm.eqb[k] : 0;
         }
     }
-    if (threadIdx.x == 0) ws.counts[gtile] = min(count, (uint32_t)S) | (sm.err << COUNT_ERR_SHIFT);
+    if (threadIdx.x == 0) ws.counts[gtile] = min(count, (uint32_t)S) | tflags | (sm.err << COUNT_ERR_SHIFT);
 }
 
-// Write-back of one dirty tile (ss_device.hpp, step 6): relabel without halo,
-// map roots of changed fragments, rewrite only their pixels.
+// Write-back of one dirty tile (ss_device.hpp, step 6).  The tile's fragments
+// become a small LDS table (changed -> new mask bits); a uniform tile is one
+// fragment, any other dirty tile has a record plane, so the pass streams the
+// plane and rewrites only the pixels of changed fragments -- no image read,
+// no relabelling.
 template <int T, int NT>
 __device__ __forceinline__ void tile_writeback(TileSmem<T>& sm, const ImgDev& im, int64_t gtile, const Ws& ws) {
-    constexpr int S = 4 * T - 4, D = Quads<T, NT>::D, PER = Quads<T, NT>::PER;
+    constexpr int D = Quads<T, NT>::D, PER = Quads<T, NT>::PER;
     const int64_t lt = gtile - im.tile_begin;
     const int tx = (int)(lt % im.tiles_x), ty = (int)(lt / im.tiles_x);
     const int x0 = tx * T, y0 = ty * T;
     const int tw = min(T, im.w - x0), th = min(T, im.h - y0);
-    for (int p = threadIdx.x; p < T * T; p += NT) sm.map[p] = 0;
-    tile_load<T, NT, false>(sm, im, x0, y0, tw, th);
-    __syncthreads();
-    const uint32_t v0 = sm.img[TileSmem<T>::P + 4];
-    uint32_t st[PER];
-    const bool mine_eq = quad_init<T, NT, false>(sm, im, x0, y0, tw, th, st, v0);
-    const bool uniform = __syncthreads_and(mine_eq) != 0;
-    if (!uniform) quad_label<T, NT, false>(sm, st);
-    const uint32_t cnt = ws.counts[gtile] & COUNT_MASK;
+    const uint32_t cw = ws.counts[gtile];
+    const uint32_t cnt = cw & COUNT_MASK;
     const uint32_t b = ws.base[gtile];
-    for (uint32_t j = threadIdx.x; j < cnt; j += NT)
-        if (ws.dbits[b + j] & DB_CHANGED) sm.map[ws.sroot[gtile * S + j]] = (uint8_t)(j + 1);
+    for (uint32_t j = threadIdx.x; j < cnt; j += NT) {
+        const uint32_t own = ws.dbits[b + j];
+        uint32_t v = 0;
+        if (own & DB_CHANGED) {
+            const uint32_t res = (ws.dbits[gfind_ro(ws.dparent, b + j)] >> 2) | own;
+            v = 0x80u | ((res & F_G) ? 0u : 1u) | ((res & F_S) ? 0u : 2u);
+        }
+        sm.lut[j] = (uint8_t)v;
+    }
     __syncthreads();
+    const bool al_out = (((uintptr_t)im.mx | (uintptr_t)im.mn | (uintptr_t)im.opitch) & 3u) == 0;
+    const bool uniform = (cw & CNT_UNIFORM) != 0;
+    const uint8_t* plane = ws.rec8 + gtile * (int64_t)(T * T);
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-        const uint32_t sel = (st[k] >> Q_VAL) & 0xFu;
-        if (!sel) continue;
-        const int q = threadIdx.x + k * NT, ly = q / D, lx0 = 4 * (q % D);
-        const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
+        const int q = threadIdx.x + k * NT;
+        if (Quads<T, NT>::Q % NT != 0 && q >= Quads<T, NT>::Q) continue;
+        const int ly = q / D, lx0 = 4 * (q % D);
+        if (ly >= th || lx0 >= tw) continue;
+        const uint32_t rcw = uniform ? 0u : *reinterpret_cast<const uint32_t*>(plane + ly * T + lx0);
+        uint32_t mxw = 0, mnw = 0, sel = 0;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            if (!((sel >> i) & 1u)) continue;
-            const uint32_t p = ly * T + lx0 + i;
-            const uint32_t root = uniform ? 0u : (((st[k] >> (Q_SGL + i)) & 1u) ? p : (sm.lab[p] & LMASK));
-            const uint32_t j1 = sm.map[root];
-            if (!j1) continue;
-            const uint32_t i_frag = b + j1 - 1;
-            const uint32_t res = (ws.dbits[ws.dparent[i_frag]] >> 2) | ws.dbits[i_frag];
-            im.mx[o + i] = (res & F_G) ? 0 : 255;
-            im.mn[o + i] = (res & F_S) ? 0 : 255;
+            const uint32_t r = (rcw >> (8 * i)) & 0xFFu;
+            if (lx0 + i >= tw || r == REC_NONE) continue;
+            const uint32_t v = sm.lut[r];
+            if (!(v & 0x80u)) continue;
+            sel |= 1u << i;
+            mxw |= ((v & 1u) ? 0xFFu : 0u) << (8 * i);
+            mnw |= ((v & 2u) ? 0xFFu : 0u) << (8 * i);
         }
+        if (!sel) continue;
+        const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
+        store_quad(im.mx + o, mxw, sel, al_out);
+        store_quad(im.mn + o, mnw, sel, al_out);
     }
     __syncthreads();
 }
