@@ -82,6 +82,11 @@ def main():
     ap.add_argument("--threads", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=None)
+    ap.add_argument("--images", default=None,
+                    help="comma-separated subset of the sweep images (for profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one device-to-device copy of one image (known bytes), so "
+                         "hardware byte counters can be scaled")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -102,7 +107,18 @@ def main():
 
     t0 = time.time()
     images = list(ssgen.sweep_images(args.side, 0, device=dev))
+    if args.images:
+        keep = set(args.images.split(","))
+        images = [(n, i) for n, i in images if n in keep]
+        assert images, "no image matches --images"
     torch.cuda.synchronize()
+    if args.copy_calibration:
+        # one plain copy of side*side bytes: its counter readings give the
+        # scale of the hardware byte counters (reads + writes = 2 * side^2 B)
+        dst = torch.empty_like(images[0][1])
+        dst.copy_(images[0][1])
+        torch.cuda.synchronize()
+        del dst
     log("generated %d images %dx%d in %.1fs" % (len(images), args.side, args.side, time.time() - t0))
 
     side = args.side

@@ -1,5 +1,6 @@
 // ss_tile.hpp -- tile-local stage of the dual regional-extrema pipeline:
-// LDS staging, Komura flat-zone labelling, evidence, fragments, masks.
+// LDS staging, Komura flat-zone labelling, evidence, fragments, masks, and the
+// record-plane write-back.
 //
 // Work mapping: a tile of T x T pixels is cut into quads of 4 horizontally
 // adjacent pixels; each thread owns the same quads through every phase and
@@ -7,6 +8,8 @@
 // so LDS holds only the image (for neighbour reads) and the shared labels.
 // LDS image rows are dword aligned (pixel (lx, ly) at (ly + 1) * P + 4 + lx),
 // so a quad reads its 3 x 6 neighbourhood with nine 32-bit LDS loads.
+// The exact seam tests (which need the image-border rules) run in a separate
+// pass over the 4T-4 perimeter pixels instead of inside every quad.
 //
 // Two shortcuts keep the common cases off the union-find:
 //   singleton pixels  no equal 8-neighbour at all: the zone is the pixel, its
@@ -19,12 +22,67 @@
 
 namespace ssx {
 
-// Per-quad register word, pixel i = 0..3 of the quad:
-constexpr int Q_FLAGS = 0;    // bits 3i..3i+2: F_G | F_S | F_SEAM
-constexpr int Q_EQL = 12;     // bit 12+2i: equal left neighbour in the tile
-constexpr int Q_EQUR = 13;    // bit 13+2i: equal up-right neighbour in the tile
-constexpr int Q_SGL = 20;     // bit 20+i: singleton (no equal 8-neighbour)
-constexpr int Q_VAL = 24;     // bit 24+i: pixel lies inside the image part of the tile
+// Per-quad register word: six 4-bit fields, bit i = pixel i of the quad.
+constexpr int Q_G = 0;        // a strictly greater 8-neighbour exists
+constexpr int Q_S = 4;        // a strictly smaller 8-neighbour exists
+constexpr int Q_EQL = 8;      // equal left neighbour in the tile
+constexpr int Q_EQUR = 12;    // equal up-right neighbour in the tile
+constexpr int Q_SGL = 16;     // singleton (no equal 8-neighbour)
+constexpr int Q_VAL = 20;     // pixel lies inside the image part of the tile
+
+// SWAR helpers on four bytes (one per pixel of a quad).
+// Top bit of every byte -> 4-bit mask.
+__device__ __forceinline__ uint32_t movemask4(uint32_t w) {
+    return (((w >> 7) & 0x01010101u) * 0x01020408u) >> 24;
+}
+// 4-bit mask -> 0x00 / 0xFF bytes.
+__device__ __forceinline__ uint32_t bytes4(uint32_t m) { return ((m * 0x00204081u) & 0x01010101u) * 0xFFu; }
+// Top bit set in every byte where x and v differ (exact, no carry between bytes).
+__device__ __forceinline__ uint32_t ne_top(uint32_t x, uint32_t v) {
+    const uint32_t z = x ^ v;
+    return (((z & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | z) & 0x80808080u;
+}
+
+// Two 16-bit lanes (v_pk_*_u16).
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_max(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(u16x2, a),
+                                                                  __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_min(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a),
+                                                                  __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_subsat(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2, a),
+                                                                      __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) + __builtin_bit_cast(u16x2, b));
+}
+// 16-bit lanes holding 0..255: top bit of the lane set iff the lane is non-zero.
+__device__ __forceinline__ uint32_t nz_top16(uint32_t a) { return pk_add(a, 0x7FFF7FFFu) & 0x80008000u; }
+
+// Horizontal 3-wide max and min of one image row around a quad, as two 16-bit
+// lane pairs: even pixels (0, 2) and odd pixels (1, 3).  x0, x1, x2 are the
+// dwords left of, at and right of the quad.  Also returns the quad's own even
+// and odd bytes.
+struct RowMinMax {
+    uint32_t maxe, maxo, mine, mino, mid_e, mid_o;
+};
+__device__ __forceinline__ RowMinMax row_minmax(uint32_t x0, uint32_t x1, uint32_t x2) {
+    RowMinMax r;
+    r.mid_e = __builtin_amdgcn_perm(0u, x1, 0x0C020C00u);       // x1.0, x1.2
+    r.mid_o = __builtin_amdgcn_perm(0u, x1, 0x0C030C01u);       // x1.1, x1.3
+    const uint32_t le = __builtin_amdgcn_perm(x1, x0, 0x0C050C03u);  // x0.3, x1.1
+    const uint32_t ro = __builtin_amdgcn_perm(x2, x1, 0x0C040C02u);  // x1.2, x2.0
+    const uint32_t mx = pk_max(r.mid_e, r.mid_o), mn = pk_min(r.mid_e, r.mid_o);
+    r.maxe = pk_max(le, mx);
+    r.maxo = pk_max(mx, ro);
+    r.mine = pk_min(le, mn);
+    r.mino = pk_min(mn, ro);
+    return r;
+}
 
 template <int T>
 struct TileSmem {
@@ -43,13 +101,29 @@ struct TileSmem {
 
 template <int T, int NT>
 struct Quads {
-    static constexpr int D = T / 4;                 // quads per row
-    static constexpr int Q = T * T / 4;             // quads per tile
-    static constexpr int PER = (Q + NT - 1) / NT;   // quads per thread
+    static constexpr int D = T / 4;                  // quads per row
+    static constexpr int Q = T * T / 4;              // quads per tile
+    static constexpr int PER = (Q + NT - 1) / NT;    // quads per thread
+    static constexpr int PPER = (4 * T + NT - 1) / NT;  // perimeter slots per thread
 };
 
 __device__ __forceinline__ uint32_t lds32(const uint8_t* p) { return *reinterpret_cast<const uint32_t*>(p); }
 
+// LDS find with path halving (label words of non-roots carry no flag bits).
+// Not for the final compression: a halving store could replace a finished
+// root entry with a mere ancestor.
+__device__ __forceinline__ uint32_t lfind_h(uint32_t* lab, uint32_t x) {
+    for (;;) {
+        const uint32_t p = lds_load(lab + x) & LMASK;
+        if (p == x) return x;
+        const uint32_t gp = lds_load(lab + p) & LMASK;
+        if (gp == p) return p;
+        __hip_atomic_store(lab + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        x = gp;
+    }
+}
+
+// Read-only LDS find; masks the evidence bits that roots accumulate.
 __device__ __forceinline__ uint32_t lfindm(const uint32_t* lab, uint32_t x) {
     uint32_t r = lds_load(lab + x) & LMASK;
     for (;;) {
@@ -59,20 +133,32 @@ __device__ __forceinline__ uint32_t lfindm(const uint32_t* lab, uint32_t x) {
     }
 }
 
-// Loads the tile (rows 0..th-1) and, with HALO, the halo rows -1 and th and the
-// halo columns -1 and tw.  Halo pixels outside the image replicate the nearest
-// image pixel: such a value belongs to the pixel itself or to one of its
-// genuine 8-neighbours, so the 3x3 greater / smaller tests equal the "absent
-// neighbour" rule exactly (equality across the border is masked separately).
-template <int T, int NT, bool HALO>
-__device__ __forceinline__ void tile_load(TileSmem<T>& sm, const ImgDev& im, int x0, int y0, int tw,
-                                          int th) {
+__device__ __forceinline__ void lunion_h(uint32_t* lab, uint32_t a, uint32_t b) {
+    for (;;) {
+        a = lfind_h(lab, a);
+        b = lfind_h(lab, b);
+        if (a == b) return;
+        const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+        uint32_t expected = hi;
+        if (__hip_atomic_compare_exchange_strong(lab + hi, &expected, lo, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+            return;
+        a = expected;
+        b = lo;
+    }
+}
+
+// Loads the tile and its halo rows -1, th and halo columns -1, tw.  Halo
+// pixels outside the image replicate the nearest image pixel: such a value
+// belongs to the pixel itself or to one of its genuine 8-neighbours, so the
+// 3x3 greater / smaller tests equal the "absent neighbour" rule exactly
+// (equality across the image border is excluded by the perimeter pass).
+template <int T, int NT>
+__device__ __forceinline__ void tile_load(TileSmem<T>& sm, const ImgDev& im, int x0, int y0, int tw, int th) {
     constexpr int P = TileSmem<T>::P, D = T / 4;
     const bool al4 = ((reinterpret_cast<uintptr_t>(im.img) | (uintptr_t)im.pitch) & 3u) == 0;
-    const int r0 = HALO ? -1 : 0;
-    const int nrows = HALO ? th + 2 : th;
-    for (int it = threadIdx.x; it < nrows * D; it += NT) {
-        const int r = r0 + it / D, c = 4 * (it % D);
+    for (int it = threadIdx.x; it < (th + 2) * D; it += NT) {
+        const int r = it / D - 1, c = 4 * (it % D);
         if (c >= tw) continue;
         int gy = y0 + r;
         gy = gy < 0 ? 0 : (gy >= im.h ? im.h - 1 : gy);
@@ -84,26 +170,25 @@ __device__ __forceinline__ void tile_load(TileSmem<T>& sm, const ImgDev& im, int
             for (int b = 0; b < 4 && c + b < tw; ++b) dst[b] = src[b];
         }
     }
-    if (HALO) {
-        for (int it = threadIdx.x; it < 2 * (th + 2); it += NT) {
-            const int r = it / 2 - 1, col = (it & 1) ? tw : -1;
-            int gy = y0 + r, gx = x0 + col;
-            gy = gy < 0 ? 0 : (gy >= im.h ? im.h - 1 : gy);
-            gx = gx < 0 ? 0 : (gx >= im.w ? im.w - 1 : gx);
-            sm.img[(r + 1) * P + 4 + col] = im.img[(int64_t)gy * im.pitch + gx];
-        }
+    for (int it = threadIdx.x; it < 2 * (th + 2); it += NT) {
+        const int r = it / 2 - 1, col = (it & 1) ? tw : -1;
+        int gy = y0 + r, gx = x0 + col;
+        gy = gy < 0 ? 0 : (gy >= im.h ? im.h - 1 : gy);
+        gx = gx < 0 ? 0 : (gx >= im.w ? im.w - 1 : gx);
+        sm.img[(r + 1) * P + 4 + col] = im.img[(int64_t)gy * im.pitch + gx];
     }
 }
 
-// First pass over the thread's quads: Komura init (link to the first equal
-// predecessor in the order up, up-left, up-right, left), the edge bits for the
-// union pass, singleton detection and, with CLASSIFY, the pixel evidence and
-// the cross-seam equality bits.  Returns whether every pixel equals v0.
-template <int T, int NT, bool CLASSIFY>
-__device__ __forceinline__ bool quad_init(TileSmem<T>& sm, const ImgDev& im, int x0, int y0, int tw,
-                                          int th, uint32_t (&st)[Quads<T, NT>::PER], uint32_t v0) {
+// First pass over the thread's quads: pixel evidence, Komura init (link to the
+// first equal predecessor in the order up, up-left, up-right, left), the edge
+// bits of the union pass and singleton detection (no equal value among the
+// eight LDS neighbours, halo included -- conservative at the image border,
+// where a replicated halo value may only make a pixel non-singleton).
+// Returns whether every pixel equals v0.
+template <int T, int NT>
+__device__ __forceinline__ bool quad_init(TileSmem<T>& sm, int tw, int th, uint32_t (&st)[Quads<T, NT>::PER],
+                                          uint32_t v0) {
     constexpr int P = TileSmem<T>::P, D = Quads<T, NT>::D, Q = Quads<T, NT>::Q;
-    const bool hasL = x0 > 0, hasR = x0 + tw < im.w, hasU = y0 > 0, hasD = y0 + th < im.h;
     bool all_eq = true;
 #pragma unroll
     for (int k = 0; k < Quads<T, NT>::PER; ++k) {
@@ -113,126 +198,157 @@ __device__ __forceinline__ bool quad_init(TileSmem<T>& sm, const ImgDev& im, int
         const int ly = q / D, lx0 = 4 * (q % D);
         if (ly >= th || lx0 >= tw) continue;
         const int base = (ly + 1) * P + 4 + lx0;
-        uint32_t U[6], C[6], L[6];
-        {
-            const uint32_t a0 = lds32(&sm.img[base - P - 4]), a1 = lds32(&sm.img[base - P]),
-                           a2 = lds32(&sm.img[base - P + 4]);
-            const uint32_t b0 = lds32(&sm.img[base - 4]), b1 = lds32(&sm.img[base]),
-                           b2 = lds32(&sm.img[base + 4]);
-            const uint32_t c0 = lds32(&sm.img[base + P - 4]), c1 = lds32(&sm.img[base + P]),
-                           c2 = lds32(&sm.img[base + P + 4]);
-            U[0] = a0 >> 24; C[0] = b0 >> 24; L[0] = c0 >> 24;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                U[b + 1] = (a1 >> (8 * b)) & 0xFFu;
-                C[b + 1] = (b1 >> (8 * b)) & 0xFFu;
-                L[b + 1] = (c1 >> (8 * b)) & 0xFFu;
-            }
-            U[5] = a2 & 0xFFu; C[5] = b2 & 0xFFu; L[5] = c2 & 0xFFu;
+        const uint32_t a0 = lds32(&sm.img[base - P - 4]), a1 = lds32(&sm.img[base - P]),
+                       a2 = lds32(&sm.img[base - P + 4]);
+        const uint32_t b0 = lds32(&sm.img[base - 4]), b1 = lds32(&sm.img[base]), b2 = lds32(&sm.img[base + 4]);
+        const uint32_t c0 = lds32(&sm.img[base + P - 4]), c1 = lds32(&sm.img[base + P]),
+                       c2 = lds32(&sm.img[base + P + 4]);
+        // Neighbour words: byte i is the neighbour of pixel i in that direction.
+        const uint32_t V = b1;
+        const uint32_t nU = ne_top(a1, V), nUL = ne_top(__builtin_amdgcn_alignbyte(a1, a0, 3), V),
+                       nUR = ne_top(__builtin_amdgcn_alignbyte(a2, a1, 1), V);
+        const uint32_t nL = ne_top(__builtin_amdgcn_alignbyte(b1, b0, 3), V),
+                       nR = ne_top(__builtin_amdgcn_alignbyte(b2, b1, 1), V);
+        const uint32_t nD = ne_top(c1, V), nDL = ne_top(__builtin_amdgcn_alignbyte(c1, c0, 3), V),
+                       nDR = ne_top(__builtin_amdgcn_alignbyte(c2, c1, 1), V);
+        // 3x3 max / min (self included: it never makes a neighbour greater or
+        // smaller) on even / odd 16-bit lanes.
+        const RowMinMax ru = row_minmax(a0, a1, a2), rc = row_minmax(b0, b1, b2), rd = row_minmax(c0, c1, c2);
+        const uint32_t ge = pk_subsat(pk_max(pk_max(ru.maxe, rc.maxe), rd.maxe), rc.mid_e);
+        const uint32_t go = pk_subsat(pk_max(pk_max(ru.maxo, rc.maxo), rd.maxo), rc.mid_o);
+        const uint32_t se = pk_subsat(rc.mid_e, pk_min(pk_min(ru.mine, rc.mine), rd.mine));
+        const uint32_t so = pk_subsat(rc.mid_o, pk_min(pk_min(ru.mino, rc.mino), rd.mino));
+        const uint32_t g4 = movemask4((nz_top16(ge) >> 8) | nz_top16(go));
+        const uint32_t s4 = movemask4((nz_top16(se) >> 8) | nz_top16(so));
+        const uint32_t val4 = tw - lx0 >= 4 ? 0xFu : ((1u << (tw - lx0)) - 1u);
+        const uint32_t lv4 = lx0 > 0 ? 0xFu : 0xEu;   // pixel has a left neighbour in the tile
+        // pixel i has a right neighbour in the tile iff lx0 + i + 1 < tw
+        const uint32_t rv4 = tw - lx0 - 1 >= 4 ? 0xFu : ((1u << (tw - lx0 - 1)) - 1u);
+        // pixels past tw hold no image data: every edge bit is masked to valid pixels
+        const uint32_t up4 = ly > 0 ? val4 : 0u;
+        const uint32_t any_eq4 = movemask4(~(nU & nUL & nUR & nL & nR & nD & nDL & nDR) & 0x80808080u);
+        const uint32_t eu4 = movemask4(nU ^ 0x80808080u) & up4;
+        const uint32_t eul4 = movemask4(nUL ^ 0x80808080u) & up4 & lv4;
+        const uint32_t eur4 = movemask4(nUR ^ 0x80808080u) & up4 & rv4;
+        const uint32_t el4 = movemask4(nL ^ 0x80808080u) & lv4 & val4;
+        all_eq = all_eq && (movemask4(ne_top(V, v0 * 0x01010101u)) & val4) == 0;
+        const uint32_t sgl4 = ~any_eq4 & val4;
+        st[k] = g4 << Q_G | s4 << Q_S | el4 << Q_EQL | eur4 << Q_EQUR | sgl4 << Q_SGL | val4 << Q_VAL;
+        uint32_t ns4 = val4 & ~sgl4;
+        while (ns4) {
+            const int i = __builtin_ctz(ns4);
+            ns4 &= ns4 - 1u;
+            const uint32_t p = ly * T + lx0 + i;
+            uint32_t parent = p;
+            if ((eu4 >> i) & 1u) parent = p - T;
+            else if ((eul4 >> i) & 1u) parent = p - T - 1;
+            else if ((eur4 >> i) & 1u) parent = p - T + 1;
+            else if ((el4 >> i) & 1u) parent = p - 1;
+            sm.lab[p] = parent;
         }
-        uint32_t s = 0;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int lx = lx0 + i;
-            if (lx >= tw) continue;
-            const int p = ly * T + lx;
-            const uint32_t v = C[i + 1];
-            const uint32_t nu = U[i + 1], nul = U[i], nur = U[i + 2], nl = C[i];
-            const uint32_t nr = C[i + 2], ndl = L[i], nd = L[i + 1], ndr = L[i + 2];
-            all_eq = all_eq && v == v0;
-            const bool eu = ly > 0 && nu == v;
-            const bool eul = ly > 0 && lx > 0 && nul == v;
-            const bool eur = ly > 0 && lx + 1 < tw && nur == v;
-            const bool el = lx > 0 && nl == v;
-            const bool er = lx + 1 < tw && nr == v;
-            const bool edl = ly + 1 < th && lx > 0 && ndl == v;
-            const bool ed = ly + 1 < th && nd == v;
-            const bool edr = ly + 1 < th && lx + 1 < tw && ndr == v;
-            s |= 1u << (Q_VAL + i);
-            uint32_t f = 0;
-            bool seam = false;
-            if (CLASSIFY) {
-                const uint32_t hi = max(max(max(nu, nul), max(nur, nl)), max(max(nr, ndl), max(nd, ndr)));
-                const uint32_t lo = min(min(min(nu, nul), min(nur, nl)), min(min(nr, ndl), min(nd, ndr)));
-                f = (hi > v ? F_G : 0u) | (lo < v ? F_S : 0u);
-                if (lx == 0 || ly == 0 || lx == tw - 1 || ly == th - 1) {
-                    // equal neighbour outside the tile but inside the image
-                    auto sq = [&](int dx, int dy, uint32_t n) -> uint32_t {
-                        const int nx = lx + dx, ny = ly + dy;
-                        const bool out = nx < 0 || nx >= tw || ny < 0 || ny >= th;
-                        const bool in_img = (nx >= 0 || hasL) && (nx < tw || hasR) && (ny >= 0 || hasU) &&
-                                            (ny < th || hasD);
-                        return (out && in_img && n == v) ? 1u : 0u;
-                    };
-                    const uint32_t sul = sq(-1, -1, nul), su = sq(0, -1, nu), sur = sq(1, -1, nur);
-                    const uint32_t sl = sq(-1, 0, nl), sr = sq(1, 0, nr);
-                    const uint32_t sdl = sq(-1, 1, ndl), sd = sq(0, 1, nd), sdr = sq(1, 1, ndr);
-                    seam = (sul | su | sur | sl | sr | sdl | sd | sdr) != 0;
-                    if (lx == tw - 1) sm.eqr[ly] = (uint8_t)(sur | (sr << 1) | (sdr << 2));
-                    if (ly == th - 1) sm.eqb[lx] = (uint8_t)(sdl | (sd << 1) | (sdr << 2));
-                    if (seam) f |= F_SEAM;
-                }
-            }
-            const bool single = !(eu | eul | eur | el | er | edl | ed | edr) && !seam;
-            s |= f << (Q_FLAGS + 3 * i);
-            if (el) s |= 1u << (Q_EQL + 2 * i);
-            if (eur) s |= 1u << (Q_EQUR + 2 * i);
-            if (single) {
-                s |= 1u << (Q_SGL + i);
-            } else {
-                uint32_t parent = p;
-                if (eu) parent = p - T;
-                else if (eul) parent = p - T - 1;
-                else if (eur) parent = p - T + 1;
-                else if (el) parent = p - 1;
-                sm.lab[p] = parent;
-            }
-        }
-        st[k] = s;
     }
     return all_eq;
 }
 
+// Perimeter slot e in [0, 4T): top row, bottom row, left and right columns
+// without their corners; every perimeter pixel appears exactly once.
+template <int T>
+__device__ __forceinline__ bool perim_pixel(int e, int tw, int th, int& lx, int& ly) {
+    const int side = e / T, k = e % T;
+    if (side == 0) { lx = k; ly = 0; return k < tw; }
+    if (side == 1) { lx = k; ly = th - 1; return k < tw && th > 1; }
+    if (side == 2) { lx = 0; ly = k; return k > 0 && k < th - 1; }
+    lx = tw - 1; ly = k;
+    return k > 0 && k < th - 1 && tw > 1;
+}
+
+// Exact seam tests of the thread's perimeter slots: an equal neighbour that is
+// outside the tile but inside the image.  Writes the cross-seam equality bits
+// of the right column and bottom row; returns one seam bit per slot.
+template <int T, int NT>
+__device__ __forceinline__ uint32_t perimeter_pass(TileSmem<T>& sm, const ImgDev& im, int x0, int y0, int tw,
+                                                   int th) {
+    constexpr int P = TileSmem<T>::P;
+    const bool hasL = x0 > 0, hasR = x0 + tw < im.w, hasU = y0 > 0, hasD = y0 + th < im.h;
+    uint32_t seam_bits = 0;
+#pragma unroll
+    for (int k = 0; k < Quads<T, NT>::PPER; ++k) {
+        const int e = threadIdx.x + k * NT;
+        int lx, ly;
+        if (e >= 4 * T || !perim_pixel<T>(e, tw, th, lx, ly)) continue;
+        const uint8_t* c = &sm.img[(ly + 1) * P + 4 + lx];
+        const uint32_t v = c[0];
+        auto sq = [&](int dx, int dy) -> uint32_t {
+            const int nx = lx + dx, ny = ly + dy;
+            const bool out = nx < 0 || nx >= tw || ny < 0 || ny >= th;
+            const bool in_img = (nx >= 0 || hasL) && (nx < tw || hasR) && (ny >= 0 || hasU) && (ny < th || hasD);
+            return (out && in_img && c[dy * P + dx] == v) ? 1u : 0u;
+        };
+        const uint32_t sul = sq(-1, -1), su = sq(0, -1), sur = sq(1, -1);
+        const uint32_t sl = sq(-1, 0), sr = sq(1, 0);
+        const uint32_t sdl = sq(-1, 1), sd = sq(0, 1), sdr = sq(1, 1);
+        if (sul | su | sur | sl | sr | sdl | sd | sdr) seam_bits |= 1u << k;
+        if (lx == tw - 1) sm.eqr[ly] = (uint8_t)(sur | (sr << 1) | (sdr << 2));
+        if (ly == th - 1) sm.eqb[lx] = (uint8_t)(sdl | (sd << 1) | (sdr << 2));
+    }
+    return seam_bits;
+}
+
 // Labels of the non-singleton pixels: compress, unite the left and up-right
-// edges (min-root CAS), compress.  With OR_FLAGS the final compression also
-// ORs each pixel's evidence into its root.  Assumes a barrier after quad_init;
-// ends with a barrier.
-template <int T, int NT, bool OR_FLAGS>
-__device__ __forceinline__ void quad_label(TileSmem<T>& sm, const uint32_t (&st)[Quads<T, NT>::PER]) {
+// edges (min-root CAS), then a read-only final compression that also ORs each
+// pixel's evidence, and the seam evidence of the perimeter slots, into the
+// roots.  Assumes a barrier after quad_init; ends with a barrier.
+template <int T, int NT>
+__device__ __forceinline__ void quad_label(TileSmem<T>& sm, const uint32_t (&st)[Quads<T, NT>::PER], int tw,
+                                           int th, uint32_t seam_bits) {
     constexpr int D = Quads<T, NT>::D;
 #pragma unroll
     for (int k = 0; k < Quads<T, NT>::PER; ++k) {
         const int q = threadIdx.x + k * NT, p0 = (q / D) * T + 4 * (q % D);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            if ((st[k] >> (Q_VAL + i)) & ~(st[k] >> (Q_SGL + i)) & 1u) sm.lab[p0 + i] = lfindm(sm.lab, p0 + i);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < Quads<T, NT>::PER; ++k) {
-        const int q = threadIdx.x + k * NT, p0 = (q / D) * T + 4 * (q % D);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if ((st[k] >> (Q_EQL + 2 * i)) & 1u) lunion(sm.lab, p0 + i, p0 + i - 1);
-            if ((st[k] >> (Q_EQUR + 2 * i)) & 1u) lunion(sm.lab, p0 + i, p0 + i - T + 1);
+        uint32_t ns4 = (st[k] >> Q_VAL) & ~(st[k] >> Q_SGL) & 0xFu;
+        while (ns4) {
+            const int i = __builtin_ctz(ns4);
+            ns4 &= ns4 - 1u;
+            sm.lab[p0 + i] = lfind_h(sm.lab, p0 + i);
         }
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < Quads<T, NT>::PER; ++k) {
         const int q = threadIdx.x + k * NT, p0 = (q / D) * T + 4 * (q % D);
+        uint32_t e = (st[k] >> Q_EQL) & 0xFFu;   // EQL bits 0-3, EQUR bits 4-7
+        while (e) {
+            const int b = __builtin_ctz(e);
+            e &= e - 1u;
+            if (b < 4) lunion_h(sm.lab, p0 + b, p0 + b - 1);
+            else lunion_h(sm.lab, p0 + b - 4, p0 + b - 4 - T + 1);
+        }
+    }
+    __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (!((st[k] >> (Q_VAL + i)) & ~(st[k] >> (Q_SGL + i)) & 1u)) continue;
+    for (int k = 0; k < Quads<T, NT>::PER; ++k) {
+        const int q = threadIdx.x + k * NT, p0 = (q / D) * T + 4 * (q % D);
+        uint32_t ns4 = (st[k] >> Q_VAL) & ~(st[k] >> Q_SGL) & 0xFu;
+        while (ns4) {
+            const int i = __builtin_ctz(ns4);
+            ns4 &= ns4 - 1u;
             const uint32_t p = p0 + i;
             const uint32_t r = lfindm(sm.lab, p);
             if (r != p) sm.lab[p] = r;
-            if (OR_FLAGS) {
-                const uint32_t f = (st[k] >> (Q_FLAGS + 3 * i)) & 7u;
-                if (f) __hip_atomic_fetch_or(&sm.lab[r], f << FLAG_SHIFT, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
+            const uint32_t f = ((st[k] >> (Q_G + i)) & 1u) | (((st[k] >> (Q_S + i)) & 1u) << 1);
+            // skip the atomic when the root already holds the evidence: every
+            // pixel of a plateau's rim would otherwise serialise on one word
+            if (f & ~(lds_load(&sm.lab[r]) >> FLAG_SHIFT))
+                __hip_atomic_fetch_or(&sm.lab[r], f << FLAG_SHIFT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
+    }
+#pragma unroll
+    for (int k = 0; k < Quads<T, NT>::PPER; ++k) {
+        if (!((seam_bits >> k) & 1u)) continue;
+        int lx, ly;
+        perim_pixel<T>(threadIdx.x + k * NT, tw, th, lx, ly);
+        const uint32_t r = lfindm(sm.lab, ly * T + lx);
+        __hip_atomic_fetch_or(&sm.lab[r], F_SEAM << FLAG_SHIFT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     __syncthreads();
 }
@@ -246,6 +362,32 @@ __device__ __forceinline__ void store_quad(uint8_t* dst, uint32_t packed, uint32
 #pragma unroll
         for (int i = 0; i < 4; ++i)
             if ((sel >> i) & 1u) dst[i] = (uint8_t)(packed >> (8 * i));
+    }
+}
+
+// Perimeter records of the thread's seam slots: fragment j in every perimeter
+// array the pixel belongs to.
+template <int T, int NT>
+__device__ __forceinline__ void write_perimeter_records(TileSmem<T>& sm, uint8_t* rec, int tw, int th,
+                                                        uint32_t seam_bits, bool uniform) {
+#pragma unroll
+    for (int k = 0; k < Quads<T, NT>::PPER; ++k) {
+        if (!((seam_bits >> k) & 1u)) continue;
+        int lx, ly;
+        perim_pixel<T>(threadIdx.x + k * NT, tw, th, lx, ly);
+        uint8_t j = 0;
+        if (!uniform) {
+            const uint32_t jr = (sm.lab[sm.lab[ly * T + lx] & LMASK] >> FRAG_SHIFT) & 0xFFu;
+            if (jr == 0u) {
+                sm.err = 2u;
+                continue;
+            }
+            j = (uint8_t)(jr - 1u);
+        }
+        if (ly == 0) rec[lx] = j;
+        if (ly == th - 1) rec[T + lx] = j;
+        if (lx == 0) rec[2 * T + ly] = j;
+        if (lx == tw - 1) rec[3 * T + ly] = j;
     }
 }
 
@@ -263,22 +405,25 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
         sm.zflags = 0;
         sm.open = 0;
     }
-    tile_load<T, NT, true>(sm, im, x0, y0, tw, th);
+    tile_load<T, NT>(sm, im, x0, y0, tw, th);
     __syncthreads();
     const uint32_t v0 = sm.img[TileSmem<T>::P + 4];
     uint32_t st[PER];
-    const bool mine_eq = quad_init<T, NT, true>(sm, im, x0, y0, tw, th, st, v0);
+    const bool mine_eq = quad_init<T, NT>(sm, tw, th, st, v0);
+    const uint32_t seam_bits = perimeter_pass<T, NT>(sm, im, x0, y0, tw, th);
     const bool uniform = __syncthreads_and(mine_eq) != 0;
     const bool al_out = (((uintptr_t)im.mx | (uintptr_t)im.mn | (uintptr_t)im.opitch) & 3u) == 0;
     uint8_t* rec = ws.rec + gtile * 4 * T;
 
     if (uniform) {
         // One zone, root 0: OR the evidence, then everything is known.
-        uint32_t f = 0;
+        uint32_t f = seam_bits ? F_SEAM : 0u;
 #pragma unroll
-        for (int k = 0; k < PER; ++k)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) f |= (st[k] >> (Q_FLAGS + 3 * i)) & 7u;
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t val4 = (st[k] >> Q_VAL) & 0xFu;
+            f |= ((st[k] >> Q_G) & val4) ? F_G : 0u;
+            f |= ((st[k] >> Q_S) & val4) ? F_S : 0u;
+        }
         if (f) atomicOr(&sm.zflags, f);
         __syncthreads();
         const uint32_t zf = sm.zflags;
@@ -295,18 +440,10 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
             const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
             store_quad(im.mx + o, mxb, sel, al_out);
             store_quad(im.mn + o, mnb, sel, al_out);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                if (!((st[k] >> (Q_FLAGS + 3 * i)) & F_SEAM)) continue;
-                const int lx = lx0 + i;
-                if (ly == 0) rec[lx] = 0;
-                if (ly == th - 1) rec[T + lx] = 0;
-                if (lx == 0) rec[2 * T + ly] = 0;
-                if (lx == tw - 1) rec[3 * T + ly] = 0;
-            }
         }
+        write_perimeter_records<T, NT>(sm, rec, tw, th, seam_bits, true);
     } else {
-        quad_label<T, NT, true>(sm, st);
+        quad_label<T, NT>(sm, st, tw, th, seam_bits);
         // Number the seam zones (their roots) and fill their fragment slots.
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
@@ -315,7 +452,7 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
             for (int i = 0; i < 4; ++i) {
                 if (!((st[k] >> (Q_VAL + i)) & ~(st[k] >> (Q_SGL + i)) & 1u)) continue;
                 const uint32_t p = p0 + i;
-                const uint32_t w = sm.lab[p];
+                const uint32_t w = lds_load(&sm.lab[p]);
                 if ((w & LMASK) != p || !((w >> FLAG_SHIFT) & F_SEAM)) continue;
                 const uint32_t j = atomicAdd(&sm.count, 1u);
                 if (j >= (uint32_t)S) {
@@ -329,9 +466,9 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
             }
         }
         __syncthreads();
-        // Masks (final for tile-contained zones, provisional for seam zones),
-        // the perimeter records of seam pixels and, when the tile holds an open
-        // fragment, its per-pixel record plane for the write-back.
+        // Masks (final for tile-contained zones, provisional for seam zones)
+        // and, when the tile holds an open fragment, its per-pixel record
+        // plane for the write-back.
         const bool open = sm.open != 0;
         uint8_t* plane = ws.rec8 + gtile * (int64_t)(T * T);
 #pragma unroll
@@ -339,40 +476,28 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
             const uint32_t sel = (st[k] >> Q_VAL) & 0xFu;
             if (!sel) continue;
             const int q = threadIdx.x + k * NT, ly = q / D, lx0 = 4 * (q % D);
-            uint32_t mxw = 0, mnw = 0, rcw = 0xFFFFFFFFu;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                if (!((sel >> i) & 1u)) continue;
-                const int lx = lx0 + i, p = ly * T + lx;
-                uint32_t zf, rw = 0;
-                if ((st[k] >> (Q_SGL + i)) & 1u) {
-                    zf = (st[k] >> (Q_FLAGS + 3 * i)) & 7u;
-                } else {
-                    rw = sm.lab[sm.lab[p] & LMASK];
-                    zf = rw >> FLAG_SHIFT;
-                    const uint32_t jr = (rw >> FRAG_SHIFT) & 0xFFu;
-                    if (jr && (zf & 3u) != 3u) rcw ^= (0xFFu ^ (jr - 1u)) << (8 * i);
-                }
+            // singletons straight from the evidence bits, the others from their root
+            const uint32_t sgl = bytes4((st[k] >> Q_SGL) & 0xFu);
+            uint32_t mxw = ~bytes4((st[k] >> Q_G) & 0xFu) & sgl, mnw = ~bytes4((st[k] >> Q_S) & 0xFu) & sgl;
+            uint32_t rcw = 0xFFFFFFFFu;
+            uint32_t ns4 = sel & ~(st[k] >> Q_SGL) & 0xFu;
+            while (ns4) {
+                const int i = __builtin_ctz(ns4);
+                ns4 &= ns4 - 1u;
+                const int p = ly * T + lx0 + i;
+                const uint32_t rw = sm.lab[sm.lab[p] & LMASK];
+                const uint32_t zf = rw >> FLAG_SHIFT;
+                const uint32_t jr = (rw >> FRAG_SHIFT) & 0xFFu;
+                if (jr && (zf & 3u) != 3u) rcw ^= (0xFFu ^ (jr - 1u)) << (8 * i);
                 mxw |= ((zf & F_G) ? 0u : 0xFFu) << (8 * i);
                 mnw |= ((zf & F_S) ? 0u : 0xFFu) << (8 * i);
-                if ((st[k] >> (Q_FLAGS + 3 * i)) & F_SEAM) {
-                    const uint32_t jr = (rw >> FRAG_SHIFT) & 0xFFu;
-                    if (jr == 0u) {
-                        sm.err = 2u;
-                        continue;
-                    }
-                    const uint8_t j = (uint8_t)(jr - 1u);
-                    if (ly == 0) rec[lx] = j;
-                    if (ly == th - 1) rec[T + lx] = j;
-                    if (lx == 0) rec[2 * T + ly] = j;
-                    if (lx == tw - 1) rec[3 * T + ly] = j;
-                }
             }
             const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
             store_quad(im.mx + o, mxw, sel, al_out);
             store_quad(im.mn + o, mnw, sel, al_out);
             if (open) *reinterpret_cast<uint32_t*>(plane + ly * T + lx0) = rcw;
         }
+        write_perimeter_records<T, NT>(sm, rec, tw, th, seam_bits, false);
     }
     __syncthreads();
     const uint32_t count = sm.count;
