@@ -61,6 +61,140 @@ def cpu_baseline(images, threads):
     return px / med / 1e9, names
 
 
+def batched_leg(n, dev, ss, ssgen, torch, reps=3):
+    """Mixed-size batch (widths and heights U[256, 1024], content 1:1:1 iid /
+    smoothed sigma 3 at 16 levels / EDT of disks): autotuned (tile, threads),
+    per-image launches of the plain path versus one hipGraph-replayed batched
+    pipeline (persistent tile stage over a device work queue)."""
+    import numpy as np
+    t0 = time.time()
+    sizes = ssgen.mixed_sizes(n, seed=0)
+    imgs = [ssgen.mixed_image(k, h, w, 0, dev) for k, (h, w) in enumerate(sizes)]
+    outs = [(torch.empty_like(i), torch.empty_like(i)) for i in imgs]
+    px = sum(h * w for h, w in sizes)
+    torch.cuda.synchronize()
+    log("batch: generated %d images (%.2f Gpx) in %.1fs" % (n, px / 1e9, time.time() - t0))
+    step = max(1, n // 64)
+    sample = list(range(0, n, step))
+    (tile, threads), _ = ss.autotune(ss.descs([imgs[i] for i in sample], [outs[i] for i in sample]), reps=2)
+    log("batch: autotuned tile=%d threads=%d" % (tile, threads))
+    stream = torch.cuda.current_stream()
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    # per-image launches of the plain path
+    ws = ss.new_workspace(max(ss.workspace_size(w, h, tile, threads) for h, w in sizes), dev)
+    for img, o in zip(imgs, outs):
+        ss.extrema(img, tile, threads, ws, o)
+    torch.cuda.synchronize()
+    plain_ms = []
+    for _ in range(reps):
+        a, b = ev(), ev()
+        a.record(stream)
+        for img, o in zip(imgs, outs):
+            ss.extrema(img, tile, threads, ws, o)
+        b.record(stream)
+        torch.cuda.synchronize()
+        plain_ms.append(a.elapsed_time(b))
+    ref = [(o[0].clone(), o[1].clone()) for o in outs[::step]]
+    del ws
+    # one graph-replayed batched pipeline
+    d = ss.descs(imgs, outs)
+    wsb = ss.new_workspace(ss.batch_workspace_size(d, tile, threads), dev)
+    plan = ss.BatchPlan(d, wsb, tile, threads)
+    for o in outs:
+        o[0].zero_()
+        o[1].zero_()
+    plan.run()
+    torch.cuda.synchronize()
+    st = ss.stats(wsb)
+    same = all(torch.equal(o[0], r[0]) and torch.equal(o[1], r[1]) for o, r in zip(outs[::step], ref))
+    batch_ms = []
+    for _ in range(reps):
+        a, b = ev(), ev()
+        a.record(stream)
+        plan.run()
+        b.record(stream)
+        torch.cuda.synchronize()
+        batch_ms.append(a.elapsed_time(b))
+    plan.close()
+    p_ms, b_ms = percentile(plain_ms, 50), percentile(batch_ms, 50)
+    return {
+        "workload": "%d images, W,H iid U[256,1024], content iid / Gaussian sigma 3 q16 / EDT of 20..300 "
+                    "disks (1:1:1), seed 0" % n,
+        "pixels": px, "tile": tile, "threads": threads,
+        "batched_gpx": round(px / (b_ms * 1e-3) / 1e9, 3), "batched_ms": round(b_ms, 3),
+        "per_image_launch_gpx": round(px / (p_ms * 1e-3) / 1e9, 3), "per_image_launch_ms": round(p_ms, 3),
+        "speedup": round(p_ms / b_ms, 3),
+        "batched_equals_per_image": bool(same), "fragments": st["fragments"], "dirty_tiles": st["dirty_tiles"],
+    }
+
+
+def serving_leg(n, dev, ss, ssgen, torch, cap_bytes=32 << 30):
+    """Open-loop Poisson requests (side log-uniform 64..8192, aspect 0.5..2,
+    the three generators), host buffers in pinned memory, HBM admission cap 32
+    GB.  Plain per-request path (H2D, one launch, D2H, in order) versus the
+    continuous-batching server at the plain path's own sustainable rate."""
+    import numpy as np
+    rng = np.random.default_rng([0, 9])
+    t0 = time.time()
+    reqs_np = []
+    keep = []
+    for k in range(n):
+        s = float(np.exp(rng.uniform(np.log(64), np.log(8192))))
+        a = float(np.exp(rng.uniform(np.log(0.5), np.log(2.0))))
+        w, h = max(1, int(round(s * a ** 0.5))), max(1, int(round(s / a ** 0.5)))
+        img = ssgen.mixed_image(k, h, w, 1, dev) if k % 3 != 2 else torch.as_tensor(ssgen.edt_disks(h, w, k))
+        host = torch.empty((h, w), dtype=torch.uint8, pin_memory=True)
+        host.copy_(img)
+        mx = torch.empty((h, w), dtype=torch.uint8, pin_memory=True)
+        mn = torch.empty((h, w), dtype=torch.uint8, pin_memory=True)
+        keep.append((host, mx, mn))
+        reqs_np.append((host.numpy(), mx.numpy(), mn.numpy()))
+    px = sum(r[0].size for r in reqs_np)
+    log("serve: generated %d requests (%.2f Gpx) in %.1fs" % (n, px / 1e9, time.time() - t0))
+    tile, threads = ss.default_config()
+    reqs = (ss.Request * n)(*[ss.Server.request(i, mx, mn, k) for k, (i, mx, mn) in enumerate(reqs_np)])
+
+    def run(server, qps, seed):
+        comps = server.run_open_loop(reqs, qps=qps, seed=seed)
+        lat = [(c.done_s - c.submit_s) * 1e3 for c in comps]
+        span = max(c.done_s for c in comps) - min(c.submit_s for c in comps)
+        return comps, lat, px / span / 1e9
+
+    # plain per-request launches: one request per launch, one stream, same
+    # arrival process (H2D, pipeline, D2H for each request in turn)
+    plain = ss.Server(hbm_cap_bytes=cap_bytes, max_batch=1, streams=1)
+    plain.wait(plain.submit(*reqs_np[0]))
+    _, _, plain_sat = run(plain, 1e9, 5)                 # capacity: all offered at once
+    qps = 0.8 * plain_sat * 1e9 / (px / n)               # 80 % of the plain path's capacity
+    _, plain_lat, plain_gpx = run(plain, qps, 11)
+    plain_ref = [(m.clone(), n_.clone()) for _, m, n_ in keep]
+    plain.close()
+    # continuous batching: launches capped at 32 Mpixel, three in flight
+    server = ss.Server(hbm_cap_bytes=cap_bytes, streams=3, max_batch_pixels=32 << 20)
+    server.wait(server.submit(*reqs_np[0]))
+    for _, m, n_ in keep:
+        m.zero_()
+        n_.zero_()
+    comps, srv_lat, srv_gpx = run(server, qps, 11)
+    same = all(torch.equal(m, r[0]) and torch.equal(n_, r[1]) for (_, m, n_), r in zip(keep, plain_ref))
+    _, _, srv_sat = run(server, 1e9, 5)
+    st = server.stats()
+    server.close()
+    return {
+        "workload": "%d requests, side log-uniform 64..8192, aspect 0.5..2, iid / sigma 3 q16 / EDT, "
+                    "pinned host buffers (H2D + pipeline + D2H per request), HBM cap %d GB, open-loop "
+                    "Poisson at 80%% of the plain path's capacity" % (n, cap_bytes >> 30),
+        "pixels": px, "qps": round(qps, 2), "tile": tile, "threads": threads,
+        "plain_p50_ms": round(percentile(plain_lat, 50), 3), "plain_p99_ms": round(percentile(plain_lat, 99), 3),
+        "plain_gpx_at_qps": round(plain_gpx, 3), "plain_saturated_gpx": round(plain_sat, 3),
+        "server_p50_ms": round(percentile(srv_lat, 50), 3), "server_p99_ms": round(percentile(srv_lat, 99), 3),
+        "server_gpx_at_qps": round(srv_gpx, 3), "server_saturated_gpx": round(srv_sat, 3),
+        "mean_batch_at_qps": round(sum(c.batch_size for c in comps) / n, 2),
+        "launches": st["launches"], "peak_inflight_bytes": st["peak_inflight_bytes"],
+        "server_equals_plain": bool(same),
+    }
+
+
 def load_traffic():
     path = os.path.join(REPO, "profiles", "pmc_tile_stage_16384.json")
     if not os.path.exists(path):
@@ -87,6 +221,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one device-to-device copy of one image (known bytes), so "
                          "hardware byte counters can be scaled")
+    ap.add_argument("--batch-images", type=int, default=4096, help="batched leg size (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=192, help="serving leg size (0 = skip)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -209,11 +345,26 @@ def main():
         "last_stats": {k: v for k, v in st.items() if k != "status"},
     }
 
+    # CPU-baseline sample (iid and sigma 4 / 16 levels when present), kept on the host
+    pick = [i for i, (n_, _) in enumerate(images) if n_ in ("iid", "smooth_s4_q16")] or [0]
+    cpu_sample = [(images[i][0], images[i][1].cpu()) for i in pick]
+    del images, mx, mn, ws
+    torch.cuda.empty_cache()
+    if args.batch_images > 0:
+        t0 = time.time()
+        out["batched"] = batched_leg(args.batch_images, dev, ss, ssgen, torch)
+        log("batched leg %.1fs" % (time.time() - t0), out["batched"])
+        torch.cuda.empty_cache()
+    if args.serve_requests > 0:
+        t0 = time.time()
+        out["serving"] = serving_leg(args.serve_requests, dev, ss, ssgen, torch)
+        log("serving leg %.1fs" % (time.time() - t0), out["serving"])
+        torch.cuda.empty_cache()
+
     if not args.no_cpu:
         cores = args.cpu_threads or min(16, os.cpu_count() or 1)
-        sample = [images[0], images[9]]  # iid and sigma-4 / 16 levels
         t0 = time.time()
-        rate, names = cpu_baseline(sample, cores)
+        rate, names = cpu_baseline(cpu_sample, cores)
         out["cpu_baseline"] = {
             "value": round(rate, 4), "unit": "Gpixel/s", "cores": cores, "kind": "port",
             "sample": "Global-UF oracle route on %s at %dx%d (median of 2 runs each after a "

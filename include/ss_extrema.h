@@ -174,6 +174,9 @@ typedef struct ss_server_config {
     int streams;             /* launches in flight, >= 1 (default 2)            */
     int tile;                /* 0 = default config                               */
     int threads;
+    uint64_t max_batch_pixels; /* stop admitting into a launch past this many
+                                  pixels (0 = no limit); smaller launches let
+                                  one launch's copies overlap another's kernels */
 } ss_server_config;
 
 typedef struct ss_request {

@@ -136,7 +136,9 @@ void ss_server::schedule_loop() {
         }
         // Admission in arrival order while the blocks fit.
         std::vector<uint64_t> ids;
-        while (!pending.empty() && (cfg.max_batch <= 0 || (int)ids.size() < cfg.max_batch)) {
+        uint64_t batch_px = 0;
+        while (!pending.empty() && (cfg.max_batch <= 0 || (int)ids.size() < cfg.max_batch) &&
+               (cfg.max_batch_pixels == 0 || batch_px < cfg.max_batch_pixels)) {
             Request& q = reqs[pending.front()];
             size_t a, b, c;
             if (!arena.alloc(q.px, &a)) break;
@@ -145,6 +147,7 @@ void ss_server::schedule_loop() {
             q.off_img = a; q.off_max = b; q.off_min = c;
             ids.push_back(pending.front());
             pending.pop_front();
+            batch_px += q.px;
         }
         std::vector<ss_image_desc> d;
         size_t ws_bytes = 0, ws_off = 0;

@@ -223,7 +223,8 @@ class BatchPlan:
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_cap_bytes", ctypes.c_uint64), ("max_batch", ctypes.c_int),
-                ("streams", ctypes.c_int), ("tile", ctypes.c_int), ("threads", ctypes.c_int)]
+                ("streams", ctypes.c_int), ("tile", ctypes.c_int), ("threads", ctypes.c_int),
+                ("max_batch_pixels", ctypes.c_uint64)]
 
 
 class Request(ctypes.Structure):
@@ -263,13 +264,13 @@ class Server:
     """Continuous-batching request loop (ss_server_*).  Requests are host
     numpy arrays; outputs are written into host arrays owned by the caller."""
 
-    def __init__(self, hbm_cap_bytes, max_batch=0, streams=2, tile=0, threads=0):
+    def __init__(self, hbm_cap_bytes, max_batch=0, streams=2, tile=0, threads=0, max_batch_pixels=0):
         lib = _lib()
         if not hasattr(lib, "_server_ready"):
             _server_protos(lib)
             lib._server_ready = True
         self._h = ctypes.c_void_p()
-        cfg = ServerConfig(hbm_cap_bytes, max_batch, streams, tile, threads)
+        cfg = ServerConfig(hbm_cap_bytes, max_batch, streams, tile, threads, max_batch_pixels)
         _check(lib.ss_server_create(ctypes.byref(cfg), ctypes.byref(self._h)), "ss_server_create")
         self._keep = {}
 
