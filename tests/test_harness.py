@@ -1,0 +1,59 @@
+"""The C++ command-line gate (sparse-seam_amd/ss_harness): exit codes, CSV,
+and the reference's oracle gate on the 206 golden triples."""
+import csv
+import os
+import subprocess
+
+import pytest
+
+from sscommon import GOLDEN, REPO, SUITES
+
+HARNESS = os.path.join(REPO, "sparse-seam_amd", "ss_harness")
+
+
+def run(args, timeout=300):
+    return subprocess.run([HARNESS] + args, capture_output=True, text=True, timeout=timeout)
+
+
+def test_usage_and_argument_errors():
+    assert os.path.exists(HARNESS)
+    assert run([]).returncode == 2
+    assert run([GOLDEN, "--bogus"]).returncode == 2
+    r = run([GOLDEN, "--skip-expected"])          # no --output
+    assert r.returncode == 3 and "Missing --output" in r.stderr
+    r = run([GOLDEN, "--output=/tmp/x", "--levels=1"])
+    assert r.returncode == 3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("suite", SUITES)
+def test_golden_gate(tmp_path, suite):
+    root = os.path.join(GOLDEN, suite)
+    out = tmp_path / "out"
+    r = run([os.path.join(root, "inputs"), "--output=%s" % out,
+             "--expected-max=%s" % os.path.join(root, "expected_max"),
+             "--expected-min=%s" % os.path.join(root, "expected_min"), "--warmup=1", "--reps=2"])
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "[SS-HARNESS-VALIDATION-OK]" in r.stdout and "mismatches=0" in r.stdout
+    rows = list(csv.DictReader(open(out / "ss_harness.csv")))
+    assert len(rows) == len(os.listdir(os.path.join(root, "inputs")))
+    assert all(row["oracle_max_mismatches"] == "0" and row["oracle_min_mismatches"] == "0" for row in rows)
+    assert (out / "environment.txt").exists()
+
+
+@pytest.mark.gpu
+def test_gate_detects_mismatch(tmp_path):
+    root = os.path.join(GOLDEN, "oracle_v0_1")
+    r = run([os.path.join(root, "inputs"), "--output=%s" % (tmp_path / "o"),
+             "--expected-max=%s" % os.path.join(root, "expected_min"),   # swapped on purpose
+             "--expected-min=%s" % os.path.join(root, "expected_max"), "--warmup=0", "--reps=1"])
+    assert r.returncode == 1
+
+
+@pytest.mark.gpu
+def test_autotune_and_levels(tmp_path):
+    root = os.path.join(GOLDEN, "oracle_v0_3_sparse_seam_adversarial")
+    r = run([os.path.join(root, "inputs"), "--output=%s" % (tmp_path / "o"), "--skip-expected",
+             "--autotune", "--levels=4", "--warmup=1", "--reps=2"])
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "[SS-AUTOTUNE]" in r.stdout and "[SS-HARNESS-MEASUREMENT-OK]" in r.stdout
