@@ -157,13 +157,18 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_top(Ws ws) {
     }
 }
 
-// Per scan block: tile bases, then the dense union-find arrays of the block's
-// fragments (one wave per tile, lanes over the tile's fragments).
+// Per scan block and part: the block's scan is recomputed by each of its
+// APPLY_PARTS workgroups (1024 counts, cheap); part 0 writes the tile bases and
+// part k initialises the dense union-find arrays of the k-th slice of the
+// block's tiles (one wave per tile, lanes over the tile's fragments), so the
+// fragment copy is spread over many CUs instead of one workgroup per block.
+constexpr int APPLY_PARTS = 16;
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(Ws ws) {
     __shared__ uint32_t s_wave[SCAN_BLOCK / 64 + 1];
     __shared__ uint32_t s_base[SCAN_CHUNK];
     __shared__ uint32_t s_cnt[SCAN_CHUNK];
-    const int64_t tb = (int64_t)blockIdx.x * SCAN_CHUNK;
+    const uint32_t blk = blockIdx.x / APPLY_PARTS, part = blockIdx.x % APPLY_PARTS;
+    const int64_t tb = (int64_t)blk * SCAN_CHUNK;
     const int64_t t0 = tb + threadIdx.x * SCAN_ITEMS;
     uint32_t c[SCAN_ITEMS];
     uint32_t s = 0;
@@ -173,19 +178,20 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(Ws ws) {
         s += c[i];
     }
     uint32_t total;
-    uint32_t run = block_excl_scan(s, s_wave, total) + ws.bsum[blockIdx.x];
+    uint32_t run = block_excl_scan(s, s_wave, total) + ws.bsum[blk];
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
         const int li = threadIdx.x * SCAN_ITEMS + i;
         s_base[li] = run;
         s_cnt[li] = c[i];
-        if (t0 + i < ws.ntiles) ws.base[t0 + i] = run;
+        if (part == 0 && t0 + i < ws.ntiles) ws.base[t0 + i] = run;
         run += c[i];
     }
     __syncthreads();
     const int S = ws.slots;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int li = wid; li < SCAN_CHUNK; li += SCAN_BLOCK / 64) {
+    constexpr int SLICE = SCAN_CHUNK / APPLY_PARTS;
+    for (int li = (int)part * SLICE + wid; li < (int)(part + 1) * SLICE; li += SCAN_BLOCK / 64) {
         const uint32_t cnt = s_cnt[li];
         if (!cnt) continue;
         const uint32_t b = s_base[li];
@@ -431,7 +437,7 @@ static hipError_t launch_all(const Images& ims, const Ws& ws, int T, int NT, boo
     const uint32_t nb = ws.nblocks;
     k_scan_reduce<<<nb, SCAN_BLOCK, 0, s>>>(ws);
     k_scan_top<<<1, SCAN_BLOCK, 0, s>>>(ws);
-    k_scan_apply<<<nb, SCAN_BLOCK, 0, s>>>(ws);
+    k_scan_apply<<<nb * APPLY_PARTS, SCAN_BLOCK, 0, s>>>(ws);
     switch (T) {
     case 16: launch_merge<16, MULTI>(ims, ws, s); break;
     case 32: launch_merge<32, MULTI>(ims, ws, s); break;

@@ -22,13 +22,36 @@
 
 namespace ssx {
 
-// Per-quad register word: six 4-bit fields, bit i = pixel i of the quad.
+// Per-quad register word: eight 4-bit fields, bit i = pixel i of the quad.
 constexpr int Q_G = 0;        // a strictly greater 8-neighbour exists
 constexpr int Q_S = 4;        // a strictly smaller 8-neighbour exists
-constexpr int Q_EQL = 8;      // equal left neighbour in the tile
-constexpr int Q_EQUR = 12;    // equal up-right neighbour in the tile
-constexpr int Q_SGL = 16;     // singleton (no equal 8-neighbour)
-constexpr int Q_VAL = 20;     // pixel lies inside the image part of the tile
+constexpr int Q_EL = 8;       // equal left neighbour in the tile
+constexpr int Q_EU = 12;      // equal up neighbour in the tile
+constexpr int Q_EUL = 16;     // equal up-left neighbour in the tile
+constexpr int Q_EUR = 20;     // equal up-right neighbour in the tile
+constexpr int Q_SGL = 24;     // singleton (no equal 8-neighbour)
+constexpr int Q_VAL = 28;     // pixel lies inside the image part of the tile
+
+// Runs: the non-singleton pixels of a quad split into maximal runs of equal
+// horizontal neighbours.  Only a run's first pixel (its head) is a union-find
+// node; the other pixels of the run point at the head and never change.  A
+// plateau quad is one node instead of four.
+__device__ __forceinline__ uint32_t q_nsg(uint32_t st) { return (st >> Q_VAL) & ~(st >> Q_SGL) & 0xFu; }
+__device__ __forceinline__ uint32_t q_heads(uint32_t st) { return q_nsg(st) & ~((st >> Q_EL) & 0xEu); }
+// Pixels of the run that starts at head h.
+__device__ __forceinline__ uint32_t q_run(uint32_t st, int h) {
+    const uint32_t c = (st >> Q_EL) & 0xEu;
+    uint32_t r = 1u << h;
+    r |= (r << 1) & c;
+    r |= (r << 1) & c;
+    r |= (r << 1) & c;
+    return r;
+}
+// Row-above contacts of run r: bit j set iff the pixel above at quad column
+// j - 1 (j = 0..5) equals the run's value.
+__device__ __forceinline__ uint32_t q_above(uint32_t st, uint32_t r) {
+    return (((st >> Q_EUL) & r) | (((st >> Q_EU) & r) << 1) | (((st >> Q_EUR) & r) << 2)) & 0x3Fu;
+}
 
 // SWAR helpers on four bytes (one per pixel of a quad).
 // Top bit of every byte -> 4-bit mask.
@@ -233,19 +256,30 @@ __device__ __forceinline__ bool quad_init(TileSmem<T>& sm, int tw, int th, uint3
         const uint32_t el4 = movemask4(nL ^ 0x80808080u) & lv4 & val4;
         all_eq = all_eq && (movemask4(ne_top(V, v0 * 0x01010101u)) & val4) == 0;
         const uint32_t sgl4 = ~any_eq4 & val4;
-        st[k] = g4 << Q_G | s4 << Q_S | el4 << Q_EQL | eur4 << Q_EQUR | sgl4 << Q_SGL | val4 << Q_VAL;
-        uint32_t ns4 = val4 & ~sgl4;
-        while (ns4) {
-            const int i = __builtin_ctz(ns4);
-            ns4 &= ns4 - 1u;
-            const uint32_t p = ly * T + lx0 + i;
-            uint32_t parent = p;
-            if ((eu4 >> i) & 1u) parent = p - T;
-            else if ((eul4 >> i) & 1u) parent = p - T - 1;
-            else if ((eur4 >> i) & 1u) parent = p - T + 1;
-            else if ((el4 >> i) & 1u) parent = p - 1;
-            sm.lab[p] = parent;
+        const uint32_t s = g4 << Q_G | s4 << Q_S | el4 << Q_EL | eu4 << Q_EU | eul4 << Q_EUL | eur4 << Q_EUR |
+                           sgl4 << Q_SGL | val4 << Q_VAL;
+        st[k] = s;
+        const uint32_t nsg = val4 & ~sgl4;
+        if (!nsg) continue;
+        // Labels of the quad in one 16-byte store: a head links to its first
+        // contact above (else to the pixel left of the quad, else to itself),
+        // every other run pixel to its head.  Singletons get a dummy entry
+        // that nothing reads.  Every link points at a smaller index.
+        const uint32_t p0 = ly * T + lx0;
+        const uint32_t hd = nsg & ~(el4 & 0xEu);
+        uint32_t lab4[4];
+        uint32_t head = p0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if ((hd >> i) & 1u) {
+                head = p0 + i;
+                const uint32_t a = q_above(s, q_run(s, i));
+                lab4[i] = a ? (p0 - T + __builtin_ctz(a) - 1u) : ((i == 0 && (el4 & 1u)) ? p0 - 1u : p0 + i);
+            } else {
+                lab4[i] = head;   // run pixel (or an unread dummy)
+            }
         }
+        *reinterpret_cast<uint4*>(&sm.lab[p0]) = make_uint4(lab4[0], lab4[1], lab4[2], lab4[3]);
     }
     return all_eq;
 }
@@ -305,39 +339,51 @@ __device__ __forceinline__ void quad_label(TileSmem<T>& sm, const uint32_t (&st)
 #pragma unroll
     for (int k = 0; k < Quads<T, NT>::PER; ++k) {
         const int q = threadIdx.x + k * NT, p0 = (q / D) * T + 4 * (q % D);
-        uint32_t ns4 = (st[k] >> Q_VAL) & ~(st[k] >> Q_SGL) & 0xFu;
-        while (ns4) {
-            const int i = __builtin_ctz(ns4);
-            ns4 &= ns4 - 1u;
+        uint32_t hd = q_heads(st[k]);
+        while (hd) {
+            const int i = __builtin_ctz(hd);
+            hd &= hd - 1u;
             sm.lab[p0 + i] = lfind_h(sm.lab, p0 + i);
         }
     }
     __syncthreads();
+    // Unions of the edges the head links do not cover: the further contact
+    // segments above a run (a segment is one zone: its pixels are equal and
+    // horizontally adjacent) and the left contact of a run that starts the quad.
 #pragma unroll
     for (int k = 0; k < Quads<T, NT>::PER; ++k) {
-        const int q = threadIdx.x + k * NT, p0 = (q / D) * T + 4 * (q % D);
-        uint32_t e = (st[k] >> Q_EQL) & 0xFFu;   // EQL bits 0-3, EQUR bits 4-7
-        while (e) {
-            const int b = __builtin_ctz(e);
-            e &= e - 1u;
-            if (b < 4) lunion_h(sm.lab, p0 + b, p0 + b - 1);
-            else lunion_h(sm.lab, p0 + b - 4, p0 + b - 4 - T + 1);
+        const int q = threadIdx.x + k * NT, ly = q / D, p0 = ly * T + 4 * (q % D);
+        uint32_t hd = q_heads(st[k]);
+        while (hd) {
+            const int i = __builtin_ctz(hd);
+            hd &= hd - 1u;
+            const uint32_t a = q_above(st[k], q_run(st[k], i));
+            uint32_t seg = a & ~(a << 1);
+            if (!seg) continue;   // linked left or alone: nothing more to unite
+            seg &= seg - 1u;      // the first segment is the head's link
+            while (seg) {
+                const int j = __builtin_ctz(seg);
+                seg &= seg - 1u;
+                lunion_h(sm.lab, p0 + i, p0 - T + j - 1);
+            }
+            if (i == 0 && ((st[k] >> Q_EL) & 1u)) lunion_h(sm.lab, p0, p0 - 1);
         }
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < Quads<T, NT>::PER; ++k) {
         const int q = threadIdx.x + k * NT, p0 = (q / D) * T + 4 * (q % D);
-        uint32_t ns4 = (st[k] >> Q_VAL) & ~(st[k] >> Q_SGL) & 0xFu;
-        while (ns4) {
-            const int i = __builtin_ctz(ns4);
-            ns4 &= ns4 - 1u;
+        uint32_t hd = q_heads(st[k]);
+        while (hd) {
+            const int i = __builtin_ctz(hd);
+            hd &= hd - 1u;
             const uint32_t p = p0 + i;
             const uint32_t r = lfindm(sm.lab, p);
             if (r != p) sm.lab[p] = r;
-            const uint32_t f = ((st[k] >> (Q_G + i)) & 1u) | (((st[k] >> (Q_S + i)) & 1u) << 1);
+            const uint32_t run = q_run(st[k], i);
+            const uint32_t f = (((st[k] >> Q_G) & run) ? F_G : 0u) | (((st[k] >> Q_S) & run) ? F_S : 0u);
             // skip the atomic when the root already holds the evidence: every
-            // pixel of a plateau's rim would otherwise serialise on one word
+            // run of a plateau's rim would otherwise serialise on one word
             if (f & ~(lds_load(&sm.lab[r]) >> FLAG_SHIFT))
                 __hip_atomic_fetch_or(&sm.lab[r], f << FLAG_SHIFT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
@@ -377,7 +423,7 @@ __device__ __forceinline__ void write_perimeter_records(TileSmem<T>& sm, uint8_t
         perim_pixel<T>(threadIdx.x + k * NT, tw, th, lx, ly);
         uint8_t j = 0;
         if (!uniform) {
-            const uint32_t jr = (sm.lab[sm.lab[ly * T + lx] & LMASK] >> FRAG_SHIFT) & 0xFFu;
+            const uint32_t jr = (sm.lab[lfindm(sm.lab, ly * T + lx)] >> FRAG_SHIFT) & 0xFFu;
             if (jr == 0u) {
                 sm.err = 2u;
                 continue;
@@ -448,9 +494,10 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int q = threadIdx.x + k * NT, p0 = (q / D) * T + 4 * (q % D);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                if (!((st[k] >> (Q_VAL + i)) & ~(st[k] >> (Q_SGL + i)) & 1u)) continue;
+            uint32_t hd = q_heads(st[k]);
+            while (hd) {
+                const int i = __builtin_ctz(hd);
+                hd &= hd - 1u;
                 const uint32_t p = p0 + i;
                 const uint32_t w = lds_load(&sm.lab[p]);
                 if ((w & LMASK) != p || !((w >> FLAG_SHIFT) & F_SEAM)) continue;
@@ -480,17 +527,17 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
             const uint32_t sgl = bytes4((st[k] >> Q_SGL) & 0xFu);
             uint32_t mxw = ~bytes4((st[k] >> Q_G) & 0xFu) & sgl, mnw = ~bytes4((st[k] >> Q_S) & 0xFu) & sgl;
             uint32_t rcw = 0xFFFFFFFFu;
-            uint32_t ns4 = sel & ~(st[k] >> Q_SGL) & 0xFu;
-            while (ns4) {
-                const int i = __builtin_ctz(ns4);
-                ns4 &= ns4 - 1u;
-                const int p = ly * T + lx0 + i;
-                const uint32_t rw = sm.lab[sm.lab[p] & LMASK];
+            uint32_t hd = q_heads(st[k]);
+            while (hd) {
+                const int i = __builtin_ctz(hd);
+                hd &= hd - 1u;
+                const uint32_t rb = bytes4(q_run(st[k], i));
+                const uint32_t rw = sm.lab[sm.lab[ly * T + lx0 + i] & LMASK];
                 const uint32_t zf = rw >> FLAG_SHIFT;
                 const uint32_t jr = (rw >> FRAG_SHIFT) & 0xFFu;
-                if (jr && (zf & 3u) != 3u) rcw ^= (0xFFu ^ (jr - 1u)) << (8 * i);
-                mxw |= ((zf & F_G) ? 0u : 0xFFu) << (8 * i);
-                mnw |= ((zf & F_S) ? 0u : 0xFFu) << (8 * i);
+                if (jr && (zf & 3u) != 3u) rcw ^= (0xFFu ^ (jr - 1u)) * 0x01010101u & rb;
+                if (!(zf & F_G)) mxw |= rb;
+                if (!(zf & F_S)) mnw |= rb;
             }
             const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
             store_quad(im.mx + o, mxw, sel, al_out);
