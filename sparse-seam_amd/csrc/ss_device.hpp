@@ -11,10 +11,13 @@
 //                   the seam zones ("fragments") and publish per perimeter
 //                   pixel its fragment and its cross-seam equality bits.
 //   2. compaction   exclusive scan of the per-tile fragment counts gives every
-//                   fragment a dense global id; the dense union-find arrays
-//                   are initialised from the per-tile slots.
-//   3. seam merge   lock-free min-root CAS union with path halving over the
-//                   dense fragment ids, one thread per seam position.
+//                   fragment a dense global id.
+//   3. seam merge   a) per block of 8 x 8 tiles: union-find in LDS over the
+//                   seams inside the block; every fragment's dense parent
+//                   becomes its block-local root (dense arrays initialised);
+//                   b) lock-free min-root CAS union with path halving over the
+//                   dense ids for the seams between blocks, one thread per
+//                   seam position.
 //   4. resolve      full compression and OR of the local evidence into roots.
 //   5. mark         fragments whose resolved evidence differs from the local
 //                   evidence are flagged; their tiles go to a dirty-tile list.
@@ -41,8 +44,11 @@ constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_CHUNK = SCAN_BLOCK * SCAN_ITEMS;  // tiles per scan block
 constexpr int NUM_XCD = 8;
+constexpr int MERGE_BT = 8;        // tiles per side of a block-local seam merge
+constexpr int MERGE_THREADS = 256;
 
-// One image of a launch set.  tile_begin is the global id of its first tile.
+// One image of a launch set.  tile_begin is the global id of its first tile,
+// block_begin that of its first merge block (MERGE_BT x MERGE_BT tiles).
 struct ImgDev {
     const uint8_t* img;
     uint8_t* mx;
@@ -51,6 +57,8 @@ struct ImgDev {
     int64_t opitch;
     int32_t w, h, tiles_x, tiles_y;
     int64_t tile_begin;
+    int64_t block_begin;
+    int32_t blocks_x, blocks_y;
 };
 
 struct Images {
@@ -102,7 +110,9 @@ struct Ws {
     uint32_t* dbits;     // [tiles * S] bits 0-1 local, 2-3 resolved, 4 changed
     uint32_t* dirty;     // [tiles] dirty tile list
     uint32_t* tile_img;  // [tiles] image index of each tile (batches only)
+    uint32_t* block_img; // [nblk] image index of each merge block (batches only)
     int64_t ntiles;
+    int64_t nblk;        // merge blocks
     uint32_t nblocks;    // scan blocks = ceil(ntiles / SCAN_CHUNK)
     int tile;            // T
     int threads;         // threads per tile
@@ -159,6 +169,34 @@ __device__ __forceinline__ void lunion(uint32_t* lab, uint32_t a, uint32_t b) {
                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
             return;
         a = hi;
+        b = lo;
+    }
+}
+
+// LDS find with path halving (plain labels, no flag bits).
+__device__ __forceinline__ uint32_t lfind_halve(uint32_t* lab, uint32_t x) {
+    for (;;) {
+        const uint32_t p = lds_load(lab + x);
+        if (p == x) return x;
+        const uint32_t gp = lds_load(lab + p);
+        if (gp == p) return p;
+        __hip_atomic_store(lab + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        x = gp;
+    }
+}
+
+// Min-root LDS union; a failed CAS resumes from the parent it returns.
+__device__ __forceinline__ void lunion_halve(uint32_t* lab, uint32_t a, uint32_t b) {
+    for (;;) {
+        a = lfind_halve(lab, a);
+        b = lfind_halve(lab, b);
+        if (a == b) return;
+        const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+        uint32_t expected = hi;
+        if (__hip_atomic_compare_exchange_strong(lab + hi, &expected, lo, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+            return;
+        a = expected;
         b = lo;
     }
 }

@@ -14,20 +14,22 @@ namespace ssx {
 // Byte offsets of the workspace regions (all 256-byte aligned).
 struct Layout {
     int64_t ntiles = 0;
+    int64_t nblk = 0;
     int tile = 0;
     int slots = 0;
     uint32_t nblocks = 0;
     size_t off_counts = 0, off_base = 0, off_bsum = 0, off_berr = 0, off_sbits = 0,
            off_rec8 = 0, off_rec = 0, off_eq = 0, off_dparent = 0, off_dbits = 0, off_dirty = 0,
-           off_descs = 0, off_tile_img = 0;
+           off_descs = 0, off_tile_img = 0, off_block_img = 0;
     size_t bytes = 0;
 };
 
 inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
-inline Layout make_layout(int64_t ntiles, int T, int ndesc) {
+inline Layout make_layout(int64_t ntiles, int64_t nblk, int T, int ndesc) {
     Layout L;
     L.ntiles = ntiles;
+    L.nblk = nblk;
     L.tile = T;
     L.slots = 4 * T - 4;
     L.nblocks = (uint32_t)((ntiles + SCAN_CHUNK - 1) / SCAN_CHUNK);
@@ -51,6 +53,7 @@ inline Layout make_layout(int64_t ntiles, int T, int ndesc) {
     L.off_dirty = take(nt * 4);
     L.off_descs = take((size_t)(ndesc > 1 ? ndesc : 0) * sizeof(ImgDev));
     L.off_tile_img = take(ndesc > 1 ? nt * 4 : 0);
+    L.off_block_img = take(ndesc > 1 ? (size_t)nblk * 4 : 0);
     L.bytes = o;
     return L;
 }
@@ -71,7 +74,9 @@ inline Ws make_ws(void* base, const Layout& L, int threads) {
     w.dbits = reinterpret_cast<uint32_t*>(b + L.off_dbits);
     w.dirty = reinterpret_cast<uint32_t*>(b + L.off_dirty);
     w.tile_img = reinterpret_cast<uint32_t*>(b + L.off_tile_img);
+    w.block_img = reinterpret_cast<uint32_t*>(b + L.off_block_img);
     w.ntiles = L.ntiles;
+    w.nblk = L.nblk;
     w.nblocks = L.nblocks;
     w.tile = L.tile;
     w.threads = threads;
@@ -88,6 +93,27 @@ inline bool config_valid(int T, int NT) {
 inline int64_t tiles_of(int w, int h, int T) {
     return (int64_t)((w + T - 1) / T) * (int64_t)((h + T - 1) / T);
 }
+
+// Descriptor of one image: tile grid and merge-block grid.
+inline ImgDev make_imgdev(const uint8_t* img, uint8_t* mx, uint8_t* mn, int64_t pitch, int64_t opitch, int w,
+                          int h, int T, int64_t tile_begin, int64_t block_begin) {
+    ImgDev d{};
+    d.img = img;
+    d.mx = mx;
+    d.mn = mn;
+    d.pitch = pitch;
+    d.opitch = opitch;
+    d.w = w;
+    d.h = h;
+    d.tiles_x = (w + T - 1) / T;
+    d.tiles_y = (h + T - 1) / T;
+    d.tile_begin = tile_begin;
+    d.block_begin = block_begin;
+    d.blocks_x = (d.tiles_x + MERGE_BT - 1) / MERGE_BT;
+    d.blocks_y = (d.tiles_y + MERGE_BT - 1) / MERGE_BT;
+    return d;
+}
+inline int64_t blocks_of(const ImgDev& d) { return (int64_t)d.blocks_x * d.blocks_y; }
 
 // Resolved config: cfg, else the process default.
 ss_config resolve_config(const ss_config* cfg);

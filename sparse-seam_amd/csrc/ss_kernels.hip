@@ -57,11 +57,13 @@ __global__ __launch_bounds__(NT) void k_tile_stage_persistent(Images ims, Ws ws,
     }
 }
 
-// Per-tile image index of a batch: one workgroup per image.
+// Per-tile and per-merge-block image index of a batch: one workgroup per image.
 __global__ __launch_bounds__(256) void k_tile_map(Images ims, Ws ws) {
     const ImgDev im = ims.table[blockIdx.x];
     const int64_t n = (int64_t)im.tiles_x * im.tiles_y;
     for (int64_t t = threadIdx.x; t < n; t += blockDim.x) ws.tile_img[im.tile_begin + t] = blockIdx.x;
+    const int64_t nb = (int64_t)im.blocks_x * im.blocks_y;
+    for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) ws.block_img[im.block_begin + b] = blockIdx.x;
 }
 
 // Resets the work queues of the persistent tile stage (graph-safe: a kernel,
@@ -157,19 +159,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_top(Ws ws) {
     }
 }
 
-// Per scan block and part: the block's scan is recomputed by each of its
-// APPLY_PARTS workgroups (1024 counts, cheap); part 0 writes the tile bases and
-// part k initialises the dense union-find arrays of the k-th slice of the
-// block's tiles (one wave per tile, lanes over the tile's fragments), so the
-// fragment copy is spread over many CUs instead of one workgroup per block.
-constexpr int APPLY_PARTS = 16;
+// Per scan block: the tile bases (dense id of each tile's first fragment).
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(Ws ws) {
     __shared__ uint32_t s_wave[SCAN_BLOCK / 64 + 1];
-    __shared__ uint32_t s_base[SCAN_CHUNK];
-    __shared__ uint32_t s_cnt[SCAN_CHUNK];
-    const uint32_t blk = blockIdx.x / APPLY_PARTS, part = blockIdx.x % APPLY_PARTS;
-    const int64_t tb = (int64_t)blk * SCAN_CHUNK;
-    const int64_t t0 = tb + threadIdx.x * SCAN_ITEMS;
+    const int64_t t0 = (int64_t)blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
     uint32_t c[SCAN_ITEMS];
     uint32_t s = 0;
 #pragma unroll
@@ -178,29 +171,180 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(Ws ws) {
         s += c[i];
     }
     uint32_t total;
-    uint32_t run = block_excl_scan(s, s_wave, total) + ws.bsum[blk];
+    uint32_t run = block_excl_scan(s, s_wave, total) + ws.bsum[blockIdx.x];
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
-        const int li = threadIdx.x * SCAN_ITEMS + i;
-        s_base[li] = run;
-        s_cnt[li] = c[i];
-        if (part == 0 && t0 + i < ws.ntiles) ws.base[t0 + i] = run;
+        if (t0 + i < ws.ntiles) ws.base[t0 + i] = run;
         run += c[i];
     }
+}
+
+// ---- block-local seam merge ----------------------------------------------
+
+// Fragments a merge block keeps in LDS (every block fits at T <= 32).
+template <int T>
+struct MergeCap {
+    static constexpr int all = MERGE_BT * MERGE_BT * (4 * T - 4);
+    static constexpr int value = all < 12288 ? all : 12288;
+};
+
+// Block tile holding local fragment l: lb[i] is tile i's first local id and
+// lb[NB] the block's total, so lb[lo] <= l < lb[lo + 1] at the end.
+__device__ __forceinline__ int block_tile_of(const uint32_t* lb, uint32_t l) {
+    int lo = 0, hi = MERGE_BT * MERGE_BT;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (lb[mid] <= l) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One workgroup per block of MERGE_BT x MERGE_BT tiles of one image.  The
+// block's fragments become a union-find in LDS, the seams inside the block
+// are merged there, and each fragment's dense parent is written as its
+// block-local root, so the global merge (which sees only the seams between
+// blocks) starts from trees of depth one.  Local ids follow the dense ids'
+// order, so a local root is also the smallest dense id of its component.
+// Also initialises the dense evidence bits.  A block with more fragments than
+// LDS holds merges its inner seams on the dense arrays instead.
+template <int T, bool MULTI>
+__global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws) {
+    constexpr int S = 4 * T - 4, NB = MERGE_BT * MERGE_BT, CAP = MergeCap<T>::value;
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    static_assert(NB == 64, "block tile scan is one wave");
+    __shared__ uint32_t par[CAP];
+    __shared__ uint32_t lb[NB + 1];
+    __shared__ uint32_t gb[NB];
+    __shared__ alignas(4) uint8_t srec[NB][4 * T];
+    __shared__ alignas(4) uint8_t seq[NB][2 * T];
+    const ImgDev im = MULTI ? ims.table[ws.block_img[blockIdx.x]] : ims.one;
+    const int64_t bl = (int64_t)blockIdx.x - im.block_begin;
+    const int tx0 = (int)(bl % im.blocks_x) * MERGE_BT, ty0 = (int)(bl / im.blocks_x) * MERGE_BT;
+    const int ntx = min(MERGE_BT, im.tiles_x - tx0), nty = min(MERGE_BT, im.tiles_y - ty0);
+    auto gtile = [&](int i) -> int64_t {
+        return im.tile_begin + (int64_t)(ty0 + i / MERGE_BT) * im.tiles_x + tx0 + i % MERGE_BT;
+    };
+    if (threadIdx.x < NB) {
+        const int i = threadIdx.x;
+        uint32_t c = 0, g = 0;
+        if (i % MERGE_BT < ntx && i / MERGE_BT < nty) {
+            const int64_t t = gtile(i);
+            c = ws.counts[t] & COUNT_MASK;
+            g = ws.base[t];
+        }
+        const uint32_t inc = wave_incl_scan(c);
+        lb[i] = inc - c;
+        gb[i] = g;
+        if (i == NB - 1) lb[NB] = inc;
+    }
     __syncthreads();
-    const int S = ws.slots;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    constexpr int SLICE = SCAN_CHUNK / APPLY_PARTS;
-    for (int li = (int)part * SLICE + wid; li < (int)(part + 1) * SLICE; li += SCAN_BLOCK / 64) {
-        const uint32_t cnt = s_cnt[li];
-        if (!cnt) continue;
-        const uint32_t b = s_base[li];
-        const int64_t slot0 = (tb + li) * S;
-        for (uint32_t j = lane; j < cnt; j += 64) {
-            ws.dparent[b + j] = b + j;
-            ws.dbits[b + j] = ws.sbits[slot0 + j];
+    const uint32_t L = lb[NB];
+    if (L == 0) return;
+    const bool in_lds = L <= (uint32_t)CAP;
+    for (uint32_t l = threadIdx.x; l < L; l += MERGE_THREADS) {
+        if (in_lds) {
+            par[l] = l;
+        } else {
+            const int t = block_tile_of(lb, l);
+            const uint32_t g = gb[t] + (l - lb[t]);
+            ws.dparent[g] = g;
         }
     }
+    // perimeter records and cross-seam bits of the block's tiles
+    for (int w = threadIdx.x; w < NB * T; w += MERGE_THREADS) {
+        const int i = w / T, d = w % T;
+        if (lb[i + 1] == lb[i]) continue;
+        reinterpret_cast<uint32_t*>(srec[i])[d] = reinterpret_cast<const uint32_t*>(ws.rec + gtile(i) * 4 * T)[d];
+        if (d < T / 2)
+            reinterpret_cast<uint32_t*>(seq[i])[d] = reinterpret_cast<const uint32_t*>(ws.eq + gtile(i) * 2 * T)[d];
+    }
+    __syncthreads();
+    // Inner seams: positions [0, NR) are right columns (seam s between block
+    // tiles (s % (BT-1), s / (BT-1)) and the tile right of it), [NR, 2 NR)
+    // bottom rows (seam s between (s % BT, s / BT) and the tile below).  A
+    // wave covers consecutive positions of one seam, so a pair equal to the
+    // previous position's (lane - 1) is skipped.
+    constexpr int NR = (MERGE_BT - 1) * MERGE_BT * T;
+    const int lane = threadIdx.x & 63;
+    bool bad = false;
+    uint32_t retries = 0;
+    for (int P0 = 0; P0 < 2 * NR; P0 += MERGE_THREADS) {
+        const int P = P0 + threadIdx.x;
+        uint32_t a = NONE, bs[3] = {NONE, NONE, NONE};
+        uint32_t ta = 0, tb[3] = {0, 0, 0};
+        const int k = P % T;
+        if (P < NR) {
+            const int sidx = P / T, ix = sidx % (MERGE_BT - 1), iy = sidx / (MERGE_BT - 1);
+            const int i = iy * MERGE_BT + ix;
+            const uint32_t ci = lb[i + 1] - lb[i];
+            const uint32_t e = (ix + 1 < ntx && iy < nty && ci) ? seq[i][k] : 0u;
+            if (e) {
+                const uint32_t ja = srec[i][3 * T + k];
+                if (ja < ci) { a = lb[i] + ja; ta = i; } else bad = true;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    if (!(e & (1u << d)) || a == NONE) continue;
+                    const int ny = k + d - 1, dy = ny < 0 ? -1 : (ny >= T ? 1 : 0);
+                    const int niy = iy + dy;
+                    if (niy < 0 || niy >= nty) continue;   // another block's seam pair
+                    const int nt = niy * MERGE_BT + ix + 1;
+                    const uint32_t jb = srec[nt][2 * T + ny - dy * T];
+                    if (jb < lb[nt + 1] - lb[nt]) { bs[d] = lb[nt] + jb; tb[d] = nt; } else bad = true;
+                }
+            }
+        } else if (P < 2 * NR) {
+            const int sidx = (P - NR) / T, ix = sidx % MERGE_BT, iy = sidx / MERGE_BT;
+            const int i = iy * MERGE_BT + ix;
+            const uint32_t ci = lb[i + 1] - lb[i];
+            const uint32_t e = (ix < ntx && iy + 1 < nty && ci) ? seq[i][T + k] : 0u;
+            if (e) {
+                const uint32_t ja = srec[i][T + k];
+                if (ja < ci) { a = lb[i] + ja; ta = i; } else bad = true;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    if (!(e & (1u << d)) || a == NONE) continue;
+                    const int nx = k + d - 1, dx = nx < 0 ? -1 : (nx >= T ? 1 : 0);
+                    const int nix = ix + dx;
+                    if (nix < 0 || nix >= ntx) continue;   // another block's seam pair
+                    const int nt = (iy + 1) * MERGE_BT + nix;
+                    const uint32_t jb = srec[nt][nx - dx * T];
+                    if (jb < lb[nt + 1] - lb[nt]) { bs[d] = lb[nt] + jb; tb[d] = nt; } else bad = true;
+                }
+            }
+        }
+        const uint32_t pa = __shfl_up(a, 1, 64);
+        uint32_t pbs[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) pbs[d] = __shfl_up(bs[d], 1, 64);
+        const bool has_prev = lane > 0 && k > 0 && pa == a;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (bs[d] == NONE) continue;
+            bool dup = (d > 0 && bs[d] == bs[d - 1]) || (d > 1 && bs[d] == bs[d - 2]);
+            if (has_prev) dup = dup || bs[d] == pbs[0] || bs[d] == pbs[1] || bs[d] == pbs[2];
+            if (dup) continue;
+            if (in_lds) {
+                lunion_halve(par, a, bs[d]);
+            } else {
+                gunion(ws.dparent, gb[ta] + (a - lb[ta]), gb[tb[d]] + (bs[d] - lb[tb[d]]), retries);
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t l = threadIdx.x; l < L; l += MERGE_THREADS) {
+        const int t = block_tile_of(lb, l);
+        const uint32_t j = l - lb[t];
+        const uint32_t g = gb[t] + j;
+        ws.dbits[g] = ws.sbits[gtile(t) * S + j];
+        if (in_lds) {
+            const uint32_t r = lfind(par, l);
+            const int tr = r == l ? t : block_tile_of(lb, r);
+            ws.dparent[g] = gb[tr] + (r - lb[tr]);
+        }
+    }
+    if (retries) atomicAdd((unsigned long long*)&ws.hdr->retries, (unsigned long long)retries);
+    if (bad) atomicOr(&ws.hdr->error, 2u);
 }
 
 // ---- seam merge -----------------------------------------------------------
@@ -222,15 +366,19 @@ __global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
     constexpr uint32_t NONE = 0xFFFFFFFFu;
     uint32_t ra = NONE, rb[3] = {NONE, NONE, NONE};   // right edge: fa and up to 3 fb
     uint32_t ba = NONE, bb[3] = {NONE, NONE, NONE};   // bottom edge
-    const uint32_t cnt = gtile < ws.ntiles ? (ws.counts[gtile] & COUNT_MASK) : 0u;
-    if (cnt != 0) {
+    if (gtile < ws.ntiles) {
         const ImgDev im = locate<MULTI>(ims, ws.tile_img, gtile);
         const int64_t lt = gtile - im.tile_begin;
         const int tx = (int)(lt % im.tiles_x), ty = (int)(lt / im.tiles_x);
+        // pairs inside one merge block were united by k_block_merge
+        const int bxm = tx % MERGE_BT, bym = ty % MERGE_BT;
+        const bool right_cross = bxm == MERGE_BT - 1 || (k == 0 && bym == 0) || (k == T - 1 && bym == MERGE_BT - 1);
+        const bool bottom_cross = bym == MERGE_BT - 1 || (k == 0 && bxm == 0) || (k == T - 1 && bxm == MERGE_BT - 1);
+        const uint32_t cnt = (right_cross || bottom_cross) ? (ws.counts[gtile] & COUNT_MASK) : 0u;
         const uint8_t* eq = ws.eq + gtile * 2 * T;
         const uint8_t* rec = ws.rec + gtile * 4 * T;
-        const uint32_t base = ws.base[gtile];
-        if (tx + 1 < im.tiles_x) {
+        const uint32_t base = cnt ? ws.base[gtile] : 0u;
+        if (cnt && right_cross && tx + 1 < im.tiles_x) {
             const uint32_t e = eq[k];
             if (e) {
                 const uint32_t ja = rec[3 * T + k];
@@ -241,6 +389,7 @@ __global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
                     if (!(e & (1u << d)) || ra == NONE) continue;
                     const int ny = y + d - 1;
                     const int nty = ny / T;
+                    if (bxm != MERGE_BT - 1 && nty / MERGE_BT == ty / MERGE_BT) continue;
                     const int64_t nt = im.tile_begin + (int64_t)nty * im.tiles_x + tx + 1;
                     const uint32_t jb = ws.rec[nt * 4 * T + 2 * T + (ny - nty * T)];
                     if (jb >= (ws.counts[nt] & COUNT_MASK)) { bad = true; continue; }
@@ -248,7 +397,7 @@ __global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
                 }
             }
         }
-        if (ty + 1 < im.tiles_y) {
+        if (cnt && bottom_cross && ty + 1 < im.tiles_y) {
             const uint32_t e = eq[T + k];
             if (e) {
                 const uint32_t ja = rec[T + k];
@@ -259,6 +408,7 @@ __global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
                     if (!(e & (1u << d)) || ba == NONE) continue;
                     const int nx = x + d - 1;
                     const int ntx = nx / T;
+                    if (bym != MERGE_BT - 1 && ntx / MERGE_BT == tx / MERGE_BT) continue;
                     const int64_t nt = im.tile_begin + (int64_t)(ty + 1) * im.tiles_x + ntx;
                     const uint32_t jb = ws.rec[nt * 4 * T + (nx - ntx * T)];
                     if (jb >= (ws.counts[nt] & COUNT_MASK)) { bad = true; continue; }
@@ -402,6 +552,7 @@ static void launch_writeback(const Images& ims, const Ws& ws, int grid_cap, hipS
 
 template <int T, bool MULTI>
 static void launch_merge(const Images& ims, const Ws& ws, hipStream_t s) {
+    k_block_merge<T, MULTI><<<(unsigned)ws.nblk, MERGE_THREADS, 0, s>>>(ims, ws);
     const int64_t merge_threads = ws.ntiles * T;
     const unsigned merge_grid = (unsigned)((merge_threads + 255) / 256);
     k_seam_merge<T, MULTI><<<merge_grid, 256, 0, s>>>(ims, ws);
@@ -437,7 +588,7 @@ static hipError_t launch_all(const Images& ims, const Ws& ws, int T, int NT, boo
     const uint32_t nb = ws.nblocks;
     k_scan_reduce<<<nb, SCAN_BLOCK, 0, s>>>(ws);
     k_scan_top<<<1, SCAN_BLOCK, 0, s>>>(ws);
-    k_scan_apply<<<nb * APPLY_PARTS, SCAN_BLOCK, 0, s>>>(ws);
+    k_scan_apply<<<nb, SCAN_BLOCK, 0, s>>>(ws);
     switch (T) {
     case 16: launch_merge<16, MULTI>(ims, ws, s); break;
     case 32: launch_merge<32, MULTI>(ims, ws, s); break;
