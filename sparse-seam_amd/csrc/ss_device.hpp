@@ -19,12 +19,11 @@
 //                   dense ids for the seams between blocks, one thread per
 //                   seam position.
 //   4. resolve      full compression and OR of the local evidence into roots.
-//   5. mark         fragments whose resolved evidence differs from the local
-//                   evidence are flagged; their tiles go to a dirty-tile list.
-//   6. write-back   dirty tiles stream their 1-byte per-pixel record plane
-//                   (written in step 1 only by tiles holding a fragment that
-//                   lacks one kind of evidence) and rewrite only the pixels of
-//                   changed fragments.
+//   5. mark and     one wave per tile holding a fragment that may still change:
+//      write-back   fragments whose resolved evidence differs from the local
+//                   evidence are found; if any, the tile streams its 1-byte
+//                   per-pixel record plane (written in step 1 only by such
+//                   tiles) and rewrites only the pixels of changed fragments.
 //
 // Everything is exact integer work; no MFMA (nothing here is a contraction).
 #pragma once
@@ -39,7 +38,6 @@ constexpr uint32_t F_SEAM = 4u;   // an equal 8-neighbour lies in another tile
 constexpr uint32_t LMASK = 0xFFFu;      // label bits (T*T <= 4096)
 constexpr int FLAG_SHIFT = 16;          // zone evidence OR'ed at the root
 constexpr int FRAG_SHIFT = 20;          // fragment id + 1 stored at seam roots
-constexpr uint32_t DB_CHANGED = 16u;    // dense bits: resolved output differs
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_CHUNK = SCAN_BLOCK * SCAN_ITEMS;  // tiles per scan block
@@ -72,7 +70,7 @@ struct WsHeader {
     uint32_t magic;
     uint32_t error;        // invariant word of the last run
     uint32_t reserved;
-    uint32_t dirty_count;
+    uint32_t dirty_count;  // tiles rewritten by the write-back
     uint32_t total;        // fragments in the dense array
     int32_t tile;
     int32_t threads;
@@ -107,8 +105,7 @@ struct Ws {
                          //   [0,T) top row, [T,2T) bottom row, [2T,3T) left col, [3T,4T) right col
     uint8_t* eq;         // [tiles * 2T] cross-seam equality bits: [0,T) right col, [T,2T) bottom row
     uint32_t* dparent;   // [tiles * S] dense union-find parents
-    uint32_t* dbits;     // [tiles * S] bits 0-1 local, 2-3 resolved, 4 changed
-    uint32_t* dirty;     // [tiles] dirty tile list
+    uint32_t* dbits;     // [tiles * S] bits 0-1 local, 2-3 resolved
     uint32_t* tile_img;  // [tiles] image index of each tile (batches only)
     uint32_t* block_img; // [nblk] image index of each merge block (batches only)
     int64_t ntiles;

@@ -19,7 +19,7 @@ struct Layout {
     int slots = 0;
     uint32_t nblocks = 0;
     size_t off_counts = 0, off_base = 0, off_bsum = 0, off_berr = 0, off_sbits = 0,
-           off_rec8 = 0, off_rec = 0, off_eq = 0, off_dparent = 0, off_dbits = 0, off_dirty = 0,
+           off_rec8 = 0, off_rec = 0, off_eq = 0, off_dparent = 0, off_dbits = 0,
            off_descs = 0, off_tile_img = 0, off_block_img = 0;
     size_t bytes = 0;
 };
@@ -50,7 +50,6 @@ inline Layout make_layout(int64_t ntiles, int64_t nblk, int T, int ndesc) {
     L.off_eq = take(nt * 2 * (size_t)T);
     L.off_dparent = take(nt * S * 4);
     L.off_dbits = take(nt * S * 4);
-    L.off_dirty = take(nt * 4);
     L.off_descs = take((size_t)(ndesc > 1 ? ndesc : 0) * sizeof(ImgDev));
     L.off_tile_img = take(ndesc > 1 ? nt * 4 : 0);
     L.off_block_img = take(ndesc > 1 ? (size_t)nblk * 4 : 0);
@@ -72,7 +71,6 @@ inline Ws make_ws(void* base, const Layout& L, int threads) {
     w.eq = reinterpret_cast<uint8_t*>(b + L.off_eq);
     w.dparent = reinterpret_cast<uint32_t*>(b + L.off_dparent);
     w.dbits = reinterpret_cast<uint32_t*>(b + L.off_dbits);
-    w.dirty = reinterpret_cast<uint32_t*>(b + L.off_dirty);
     w.tile_img = reinterpret_cast<uint32_t*>(b + L.off_tile_img);
     w.block_img = reinterpret_cast<uint32_t*>(b + L.off_block_img);
     w.ntiles = L.ntiles;

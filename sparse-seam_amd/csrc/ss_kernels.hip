@@ -6,7 +6,7 @@
 //   classify / number / emit        :291-475  -> tile_stage()
 //   seam merge (3 pairs per seam px):508-581  -> k_seam_merge
 //   compress + reduce               :583-619  -> k_resolve
-//   materialise seam records        :477-506  -> k_mark + k_writeback
+//   materialise seam records        :477-506  -> k_writeback
 // The reference writes a 1-byte record for every pixel and re-reads it for
 // every pixel; here masks are written once (provisionally for seam zones), the
 // record plane is written only by tiles holding a fragment that can still
@@ -216,8 +216,8 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
     __shared__ uint32_t par[CAP];
     __shared__ uint32_t lb[NB + 1];
     __shared__ uint32_t gb[NB];
-    __shared__ alignas(4) uint8_t srec[NB][4 * T];
-    __shared__ alignas(4) uint8_t seq[NB][2 * T];
+    __shared__ alignas(16) uint8_t srec[NB][4 * T];
+    __shared__ alignas(16) uint8_t seq[NB][2 * T];
     const ImgDev im = MULTI ? ims.table[ws.block_img[blockIdx.x]] : ims.one;
     const int64_t bl = (int64_t)blockIdx.x - im.block_begin;
     const int tx0 = (int)(bl % im.blocks_x) * MERGE_BT, ty0 = (int)(bl / im.blocks_x) * MERGE_BT;
@@ -242,22 +242,43 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
     const uint32_t L = lb[NB];
     if (L == 0) return;
     const bool in_lds = L <= (uint32_t)CAP;
-    for (uint32_t l = threadIdx.x; l < L; l += MERGE_THREADS) {
-        if (in_lds) {
-            par[l] = l;
-        } else {
-            const int t = block_tile_of(lb, l);
-            const uint32_t g = gb[t] + (l - lb[t]);
-            ws.dparent[g] = g;
+    // one wave per block tile, lanes over its fragments
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int t = wid; t < NB; t += MERGE_THREADS / 64) {
+        const uint32_t l0 = lb[t], c = lb[t + 1] - l0;
+        for (uint32_t j = lane; j < c; j += 64) {
+            if (in_lds) {
+                par[l0 + j] = l0 + j;
+            } else {
+                ws.dparent[gb[t] + j] = gb[t] + j;
+            }
         }
     }
-    // perimeter records and cross-seam bits of the block's tiles
-    for (int w = threadIdx.x; w < NB * T; w += MERGE_THREADS) {
-        const int i = w / T, d = w % T;
-        if (lb[i + 1] == lb[i]) continue;
-        reinterpret_cast<uint32_t*>(srec[i])[d] = reinterpret_cast<const uint32_t*>(ws.rec + gtile(i) * 4 * T)[d];
-        if (d < T / 2)
-            reinterpret_cast<uint32_t*>(seq[i])[d] = reinterpret_cast<const uint32_t*>(ws.eq + gtile(i) * 2 * T)[d];
+    // Perimeter records (4T bytes) and cross-seam bits (2T bytes) of the
+    // block's tiles, 16 bytes per load, all loads of a thread issued together.
+    {
+        constexpr int RV = T / 4, EV = T / 8;   // 16-byte vectors per tile
+        constexpr int NV = NB * (RV + EV);
+        constexpr int PER = (NV + MERGE_THREADS - 1) / MERGE_THREADS;
+        uint4 v[PER];
+#pragma unroll
+        for (int m = 0; m < PER; ++m) {
+            const int w = threadIdx.x + m * MERGE_THREADS;
+            v[m] = make_uint4(0, 0, 0, 0);
+            if (w >= NV) continue;
+            const int i = w / (RV + EV), d = w % (RV + EV);
+            if (lb[i + 1] == lb[i]) continue;
+            v[m] = d < RV ? reinterpret_cast<const uint4*>(ws.rec + gtile(i) * 4 * T)[d]
+                          : reinterpret_cast<const uint4*>(ws.eq + gtile(i) * 2 * T)[d - RV];
+        }
+#pragma unroll
+        for (int m = 0; m < PER; ++m) {
+            const int w = threadIdx.x + m * MERGE_THREADS;
+            if (w >= NV) continue;
+            const int i = w / (RV + EV), d = w % (RV + EV);
+            if (d < RV) reinterpret_cast<uint4*>(srec[i])[d] = v[m];
+            else reinterpret_cast<uint4*>(seq[i])[d - RV] = v[m];
+        }
     }
     __syncthreads();
     // Inner seams: positions [0, NR) are right columns (seam s between block
@@ -266,9 +287,9 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
     // wave covers consecutive positions of one seam, so a pair equal to the
     // previous position's (lane - 1) is skipped.
     constexpr int NR = (MERGE_BT - 1) * MERGE_BT * T;
-    const int lane = threadIdx.x & 63;
     bool bad = false;
     uint32_t retries = 0;
+#pragma unroll 1
     for (int P0 = 0; P0 < 2 * NR; P0 += MERGE_THREADS) {
         const int P = P0 + threadIdx.x;
         uint32_t a = NONE, bs[3] = {NONE, NONE, NONE};
@@ -332,15 +353,18 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
         }
     }
     __syncthreads();
-    for (uint32_t l = threadIdx.x; l < L; l += MERGE_THREADS) {
-        const int t = block_tile_of(lb, l);
-        const uint32_t j = l - lb[t];
-        const uint32_t g = gb[t] + j;
-        ws.dbits[g] = ws.sbits[gtile(t) * S + j];
-        if (in_lds) {
-            const uint32_t r = lfind(par, l);
-            const int tr = r == l ? t : block_tile_of(lb, r);
-            ws.dparent[g] = gb[tr] + (r - lb[tr]);
+    for (int t = wid; t < NB; t += MERGE_THREADS / 64) {
+        const uint32_t l0 = lb[t], c = lb[t + 1] - l0;
+        if (!c) continue;
+        const uint8_t* sb = ws.sbits + gtile(t) * S;
+        for (uint32_t j = lane; j < c; j += 64) {
+            const uint32_t g = gb[t] + j;
+            ws.dbits[g] = sb[j];
+            if (in_lds) {
+                const uint32_t r = lfind(par, l0 + j);   // r <= l0 + j
+                const int tr = r >= l0 ? t : block_tile_of(lb, r);
+                ws.dparent[g] = gb[tr] + (r - lb[tr]);
+            }
         }
     }
     if (retries) atomicAdd((unsigned long long*)&ws.hdr->retries, (unsigned long long)retries);
@@ -483,49 +507,31 @@ __global__ __launch_bounds__(256) void k_resolve(Ws ws) {
     }
 }
 
-// ---- mark: fragments whose output changes, dirty-tile list ---------------
+// ---- mark + write-back of changed seam zones -----------------------------
 
-__global__ __launch_bounds__(256) void k_mark(Ws ws) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool dirty = false;
-    if (t < ws.ntiles) {
-        const uint32_t cnt = ws.counts[t] & COUNT_MASK;
-        const uint32_t b = ws.base[t];
-        for (uint32_t j = 0; j < cnt; ++j) {
-            const uint32_t i = b + j;
-            const uint32_t own = ws.dbits[i];
-            const uint32_t res = (ws.dbits[gfind_ro(ws.dparent, i)] >> 2) & 3u;
-            if (res & ~own & 3u) {
-                ws.dbits[i] = own | DB_CHANGED;
-                dirty = true;
-            }
-        }
+// One wave per tile, a fixed grid of waves over contiguous tile ranges (the
+// ranges of one XCD's waves are adjacent, so neighbouring tiles share an L2).
+// Tiles without an open fragment are skipped on their counts word alone.
+template <int T, bool MULTI>
+__global__ __launch_bounds__(256) void k_writeback(Images ims, Ws ws) {
+    __shared__ uint8_t lut[4][4 * T];
+    __shared__ uint32_t s_dirty;
+    if (threadIdx.x == 0) s_dirty = 0;
+    __syncthreads();
+    const int wid = threadIdx.x >> 6;
+    const int64_t nw = (int64_t)gridDim.x * 4;
+    const int64_t w = xcd_block(blockIdx.x, gridDim.x) * 4 + wid;
+    const int64_t t0 = ws.ntiles * w / nw, t1 = ws.ntiles * (w + 1) / nw;
+    uint32_t nd = 0;
+    for (int64_t t = t0; t < t1; ++t) {
+        const uint32_t cw = ws.counts[t];
+        if (!(cw & COUNT_MASK) || !(cw & (CNT_OPEN | CNT_UNIFORM))) continue;
+        const ImgDev im = locate<MULTI>(ims, ws.tile_img, t);
+        nd += tile_writeback<T>(lut[wid], im, t, cw, ws) ? 1u : 0u;
     }
-    const uint64_t m = __ballot(dirty);
-    if (!m) return;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t slot0 = 0;
-    if (lane == leader) slot0 = atomicAdd(&ws.hdr->dirty_count, (uint32_t)__popcll(m));
-    slot0 = __shfl(slot0, leader, 64);
-    if (dirty) {
-        const uint32_t rank = __popcll(m & ((1ull << lane) - 1ull));
-        ws.dirty[slot0 + rank] = (uint32_t)t;
-    }
-}
-
-// ---- write-back of changed seam zones ------------------------------------
-
-template <int T, int NT, bool MULTI>
-__global__ __launch_bounds__(NT) void k_writeback(Images ims, Ws ws) {
-    __shared__ TileSmem<T> sm;
-    const uint32_t nd = ws.hdr->dirty_count;
-#pragma unroll 1
-    for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
-        const int64_t gtile = ws.dirty[d];
-        const ImgDev im = locate<MULTI>(ims, ws.tile_img, gtile);
-        tile_writeback<T, NT>(sm, im, gtile, ws);
-    }
+    if ((threadIdx.x & 63) == 0 && nd) atomicAdd(&s_dirty, nd);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_dirty) atomicAdd(&ws.hdr->dirty_count, s_dirty);
 }
 
 // ---- host sequencer -------------------------------------------------------
@@ -544,10 +550,10 @@ static void launch_tile_kernels(const Images& ims, const Ws& ws, bool persistent
     }
 }
 
-template <int T, int NT, bool MULTI>
+template <int T, bool MULTI>
 static void launch_writeback(const Images& ims, const Ws& ws, int grid_cap, hipStream_t s) {
-    const int64_t grid = std::min<int64_t>(ws.ntiles, grid_cap);
-    k_writeback<T, NT, MULTI><<<(unsigned)grid, NT, 0, s>>>(ims, ws);
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((ws.ntiles + 3) / 4, grid_cap));
+    k_writeback<T, MULTI><<<(unsigned)grid, 256, 0, s>>>(ims, ws);
 }
 
 template <int T, bool MULTI>
@@ -599,10 +605,12 @@ static hipError_t launch_all(const Images& ims, const Ws& ws, int T, int NT, boo
     const unsigned resolve_grid =
         (unsigned)std::max<int64_t>(1, std::min<int64_t>((max_frag + 255) / 256, (int64_t)grid_cap * 2));
     k_resolve<<<resolve_grid, 256, 0, s>>>(ws);
-    k_mark<<<(unsigned)((ws.ntiles + 255) / 256), 256, 0, s>>>(ws);
-#define CALL_WB(a, b) launch_writeback<a, b, MULTI>(ims, ws, grid_cap, s)
-    SSX_DISPATCH(T, NT, CALL_WB)
-#undef CALL_WB
+    switch (T) {
+    case 16: launch_writeback<16, MULTI>(ims, ws, grid_cap, s); break;
+    case 32: launch_writeback<32, MULTI>(ims, ws, grid_cap, s); break;
+    case 64: launch_writeback<64, MULTI>(ims, ws, grid_cap, s); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

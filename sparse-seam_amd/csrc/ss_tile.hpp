@@ -559,58 +559,88 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
     if (threadIdx.x == 0) ws.counts[gtile] = min(count, (uint32_t)S) | tflags | (sm.err << COUNT_ERR_SHIFT);
 }
 
-// Write-back of one dirty tile (ss_device.hpp, step 6).  The tile's fragments
-// become a small LDS table (changed -> new mask bits); a uniform tile is one
-// fragment, any other dirty tile has a record plane, so the pass streams the
-// plane and rewrites only the pixels of changed fragments -- no image read,
-// no relabelling.
-template <int T, int NT>
-__device__ __forceinline__ void tile_writeback(TileSmem<T>& sm, const ImgDev& im, int64_t gtile, const Ws& ws) {
-    constexpr int D = Quads<T, NT>::D, PER = Quads<T, NT>::PER;
+// Mark and write-back of one tile by one wave (ss_device.hpp, step 5).
+// cw is the tile's counts word; only a tile with an open fragment, or a
+// uniform tile (whose one fragment may be open), can change.  Lanes resolve
+// the tile's fragments against the seam union-find; a fragment whose resolved
+// evidence adds a bit to its local evidence changed, and its final mask bits
+// go into the wave's LDS table.  If any changed, the wave rewrites only the
+// pixels of changed fragments: a uniform tile is one fragment, any other tile
+// streams its record plane -- no image read, no relabelling.  Returns whether
+// the tile changed.
+template <int T>
+__device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, int64_t gtile, uint32_t cw,
+                                               const Ws& ws) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t cnt = cw & COUNT_MASK;
+    const uint32_t b = ws.base[gtile];
+    bool changed = false;
+    for (uint32_t j = lane; j < cnt; j += 64) {
+        const uint32_t own = ws.dbits[b + j];
+        uint32_t v = 0;
+        if ((own & 3u) != 3u) {
+            const uint32_t res = ((ws.dbits[gfind_ro(ws.dparent, b + j)] >> 2) | own) & 3u;
+            if (res & ~own) {
+                v = 0x80u | ((res & F_G) ? 0u : 1u) | ((res & F_S) ? 0u : 2u);
+                changed = true;
+            }
+        }
+        lut[j] = (uint8_t)v;
+    }
+    if (!__ballot(changed)) return false;
+    // the table was written by other lanes of this wave: LDS operations of one
+    // wave complete in order; the fence keeps the compiler from reordering
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int64_t lt = gtile - im.tile_begin;
     const int tx = (int)(lt % im.tiles_x), ty = (int)(lt / im.tiles_x);
     const int x0 = tx * T, y0 = ty * T;
     const int tw = min(T, im.w - x0), th = min(T, im.h - y0);
-    const uint32_t cw = ws.counts[gtile];
-    const uint32_t cnt = cw & COUNT_MASK;
-    const uint32_t b = ws.base[gtile];
-    for (uint32_t j = threadIdx.x; j < cnt; j += NT) {
-        const uint32_t own = ws.dbits[b + j];
-        uint32_t v = 0;
-        if (own & DB_CHANGED) {
-            const uint32_t res = (ws.dbits[gfind_ro(ws.dparent, b + j)] >> 2) | own;
-            v = 0x80u | ((res & F_G) ? 0u : 1u) | ((res & F_S) ? 0u : 2u);
-        }
-        sm.lut[j] = (uint8_t)v;
-    }
-    __syncthreads();
     const bool al_out = (((uintptr_t)im.mx | (uintptr_t)im.mn | (uintptr_t)im.opitch) & 3u) == 0;
-    const bool uniform = (cw & CNT_UNIFORM) != 0;
-    const uint8_t* plane = ws.rec8 + gtile * (int64_t)(T * T);
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int q = threadIdx.x + k * NT;
-        if (Quads<T, NT>::Q % NT != 0 && q >= Quads<T, NT>::Q) continue;
-        const int ly = q / D, lx0 = 4 * (q % D);
-        if (ly >= th || lx0 >= tw) continue;
-        const uint32_t rcw = uniform ? 0u : *reinterpret_cast<const uint32_t*>(plane + ly * T + lx0);
-        uint32_t mxw = 0, mnw = 0, sel = 0;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const uint32_t r = (rcw >> (8 * i)) & 0xFFu;
-            if (lx0 + i >= tw || r == REC_NONE) continue;
-            const uint32_t v = sm.lut[r];
-            if (!(v & 0x80u)) continue;
-            sel |= 1u << i;
-            mxw |= ((v & 1u) ? 0xFFu : 0u) << (8 * i);
-            mnw |= ((v & 2u) ? 0xFFu : 0u) << (8 * i);
+    if (cw & CNT_UNIFORM) {
+        const uint32_t v = lut[0];
+        const uint32_t mxb = (v & 1u) ? 0xFFFFFFFFu : 0u, mnb = (v & 2u) ? 0xFFFFFFFFu : 0u;
+        for (int q = lane; q < T * T / 4; q += 64) {
+            const int ly = q / (T / 4), lx0 = 4 * (q % (T / 4));
+            if (ly >= th || lx0 >= tw) continue;
+            const uint32_t sel = tw - lx0 >= 4 ? 0xFu : ((1u << (tw - lx0)) - 1u);
+            const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
+            store_quad(im.mx + o, mxb, sel, al_out);
+            store_quad(im.mn + o, mnb, sel, al_out);
         }
-        if (!sel) continue;
-        const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
-        store_quad(im.mx + o, mxw, sel, al_out);
-        store_quad(im.mn + o, mnw, sel, al_out);
+        return true;
     }
-    __syncthreads();
+    // record plane: 16 bytes per lane and step, 4 quads each
+    const uint8_t* plane = ws.rec8 + gtile * (int64_t)(T * T);
+    constexpr int VR = T / 16;   // 16-byte vectors per row
+    for (int u = lane; u < T * T / 16; u += 64) {
+        const int ly = u / VR, c0 = 16 * (u % VR);
+        if (ly >= th || c0 >= tw) continue;
+        const uint4 rv = *reinterpret_cast<const uint4*>(plane + ly * T + c0);
+        const uint32_t rw[4] = {rv.x, rv.y, rv.z, rv.w};
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const int lx0 = c0 + 4 * d;
+            if (lx0 >= tw) break;
+            uint32_t mxw = 0, mnw = 0, sel = 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t r = (rw[d] >> (8 * i)) & 0xFFu;
+                if (lx0 + i >= tw || r == REC_NONE) continue;
+                const uint32_t v = lut[r];
+                if (!(v & 0x80u)) continue;
+                sel |= 1u << i;
+                mxw |= ((v & 1u) ? 0xFFu : 0u) << (8 * i);
+                mnw |= ((v & 2u) ? 0xFFu : 0u) << (8 * i);
+            }
+            if (!sel) continue;
+            const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
+            store_quad(im.mx + o, mxw, sel, al_out);
+            store_quad(im.mn + o, mnw, sel, al_out);
+        }
+    }
+    return true;
 }
 
 }  // namespace ssx
