@@ -38,6 +38,7 @@ constexpr uint32_t F_SEAM = 4u;   // an equal 8-neighbour lies in another tile
 constexpr uint32_t LMASK = 0xFFFu;      // label bits (T*T <= 4096)
 constexpr int FLAG_SHIFT = 16;          // zone evidence OR'ed at the root
 constexpr int FRAG_SHIFT = 20;          // fragment id + 1 stored at seam roots
+constexpr uint32_t DB_BROOT = 16u;      // dense bits: root of a block-local component
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_CHUNK = SCAN_BLOCK * SCAN_ITEMS;  // tiles per scan block
@@ -105,7 +106,8 @@ struct Ws {
                          //   [0,T) top row, [T,2T) bottom row, [2T,3T) left col, [3T,4T) right col
     uint8_t* eq;         // [tiles * 2T] cross-seam equality bits: [0,T) right col, [T,2T) bottom row
     uint32_t* dparent;   // [tiles * S] dense union-find parents
-    uint32_t* dbits;     // [tiles * S] bits 0-1 local, 2-3 resolved
+    uint32_t* dbits;     // [tiles * S] bits 0-1 local, 2-3 component OR (roots),
+                         //   4 block root
     uint32_t* tile_img;  // [tiles] image index of each tile (batches only)
     uint32_t* block_img; // [nblk] image index of each merge block (batches only)
     int64_t ntiles;

@@ -206,8 +206,9 @@ __device__ __forceinline__ int block_tile_of(const uint32_t* lb, uint32_t l) {
 // block-local root, so the global merge (which sees only the seams between
 // blocks) starts from trees of depth one.  Local ids follow the dense ids'
 // order, so a local root is also the smallest dense id of its component.
-// Also initialises the dense evidence bits.  A block with more fragments than
-// LDS holds merges its inner seams on the dense arrays instead.
+// Also initialises the dense evidence bits: a fragment's own local evidence,
+// and at a block root the OR over its block component.  A block with more
+// fragments than LDS holds merges its inner seams on the dense arrays instead.
 template <int T, bool MULTI>
 __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws) {
     constexpr int S = 4 * T - 4, NB = MERGE_BT * MERGE_BT, CAP = MergeCap<T>::value;
@@ -218,6 +219,8 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
     __shared__ uint32_t gb[NB];
     __shared__ alignas(16) uint8_t srec[NB][4 * T];
     __shared__ alignas(16) uint8_t seq[NB][2 * T];
+    __shared__ uint32_t own2[CAP / 16];   // 2-bit local evidence per fragment
+    __shared__ uint32_t acc2[CAP / 16];   // 2-bit OR over the component, at its root
     const ImgDev im = MULTI ? ims.table[ws.block_img[blockIdx.x]] : ims.one;
     const int64_t bl = (int64_t)blockIdx.x - im.block_begin;
     const int tx0 = (int)(bl % im.blocks_x) * MERGE_BT, ty0 = (int)(bl / im.blocks_x) * MERGE_BT;
@@ -242,15 +245,39 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
     const uint32_t L = lb[NB];
     if (L == 0) return;
     const bool in_lds = L <= (uint32_t)CAP;
-    // one wave per block tile, lanes over its fragments
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int t = wid; t < NB; t += MERGE_THREADS / 64) {
-        const uint32_t l0 = lb[t], c = lb[t + 1] - l0;
-        for (uint32_t j = lane; j < c; j += 64) {
+    const int lane = threadIdx.x & 63;
+    for (uint32_t w = threadIdx.x; w < CAP / 16; w += MERGE_THREADS) {
+        own2[w] = 0;
+        acc2[w] = 0;
+    }
+    __syncthreads();
+    // Fragment init, four fragments per thread and pass so that their
+    // evidence loads are in flight together.
+    for (uint32_t l0 = 0; l0 < L; l0 += 4 * MERGE_THREADS) {
+        uint32_t sb[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t l = l0 + u * MERGE_THREADS + threadIdx.x;
+            sb[u] = 0;
+            if (l < L) {
+                const int t = block_tile_of(lb, l);
+                sb[u] = ws.sbits[gtile(t) * S + (l - lb[t])] & 3u;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t l = l0 + u * MERGE_THREADS + threadIdx.x;
+            if (l >= L) continue;
             if (in_lds) {
-                par[l0 + j] = l0 + j;
+                par[l] = l;
+                if (sb[u]) atomicOr(&own2[l >> 4], sb[u] << (2 * (l & 15u)));
             } else {
-                ws.dparent[gb[t] + j] = gb[t] + j;
+                // every fragment is its own block root; the inner seams are
+                // merged on the dense arrays below
+                const int t = block_tile_of(lb, l);
+                const uint32_t g = gb[t] + (l - lb[t]);
+                ws.dparent[g] = g;
+                ws.dbits[g] = sb[u] | (sb[u] << 2) | DB_BROOT;
             }
         }
     }
@@ -353,18 +380,27 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
         }
     }
     __syncthreads();
-    for (int t = wid; t < NB; t += MERGE_THREADS / 64) {
-        const uint32_t l0 = lb[t], c = lb[t + 1] - l0;
-        if (!c) continue;
-        const uint8_t* sb = ws.sbits + gtile(t) * S;
-        for (uint32_t j = lane; j < c; j += 64) {
-            const uint32_t g = gb[t] + j;
-            ws.dbits[g] = sb[j];
-            if (in_lds) {
-                const uint32_t r = lfind(par, l0 + j);   // r <= l0 + j
-                const int tr = r >= l0 ? t : block_tile_of(lb, r);
-                ws.dparent[g] = gb[tr] + (r - lb[tr]);
-            }
+    if (in_lds) {
+        // evidence of each block component at its root; labels compressed
+        for (uint32_t l = threadIdx.x; l < L; l += MERGE_THREADS) {
+            const uint32_t r = lfind(par, l);
+            if (r != l) par[l] = r;
+            const uint32_t b = (own2[l >> 4] >> (2 * (l & 15u))) & 3u;
+            if (b & ~(lds_load(&acc2[r >> 4]) >> (2 * (r & 15u))))
+                atomicOr(&acc2[r >> 4], b << (2 * (r & 15u)));
+        }
+        __syncthreads();
+        // dense arrays: parent = block root; a block root also keeps its
+        // component's evidence in bits 2-3
+        for (uint32_t l = threadIdx.x; l < L; l += MERGE_THREADS) {
+            const int t = block_tile_of(lb, l);
+            const uint32_t g = gb[t] + (l - lb[t]);
+            const uint32_t r = par[l];
+            const int tr = r >= lb[t] ? t : block_tile_of(lb, r);
+            ws.dparent[g] = gb[tr] + (r - lb[tr]);
+            uint32_t d = (own2[l >> 4] >> (2 * (l & 15u))) & 3u;
+            if (r == l) d |= (((acc2[l >> 4] >> (2 * (l & 15u))) & 3u) << 2) | DB_BROOT;
+            ws.dbits[g] = d;
         }
     }
     if (retries) atomicAdd((unsigned long long*)&ws.hdr->retries, (unsigned long long)retries);
@@ -470,7 +506,10 @@ __global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
     if (bad) atomicOr(&ws.hdr->error, 2u);
 }
 
-// ---- resolve: compress + OR local evidence into the roots ----------------
+// ---- resolve: compress + OR block evidence into the roots ----------------
+
+// Only block roots take part: every other fragment's parent is its block root,
+// which carries the evidence of its whole block component.
 
 // A component can hold millions of fragments (a near-percolating plateau), so
 // OR-ing every fragment's evidence into the one root word would serialise on
@@ -483,13 +522,14 @@ __global__ __launch_bounds__(256) void k_resolve(Ws ws) {
     for (uint32_t wbase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < total; wbase += stride) {
         const uint32_t i = wbase + lane;
         uint32_t r = 0, lb = 0;
-        if (i < total) {
+        const uint32_t d = i < total ? ws.dbits[i] : 0u;
+        if (d & DB_BROOT) {
             // halving walk: compresses the trees for the other threads; a later
             // halving store may replace this entry with an ancestor, so
             // consumers finish the walk with gfind_ro
             r = gfind(ws.dparent, i);
             if (r != i) ws.dparent[i] = r;
-            lb = ws.dbits[i] & 3u;
+            lb = (d >> 2) & 3u;
             if (lb && ((ws.dbits[r] >> 2) & lb) == lb) lb = 0;
         }
         uint64_t pending = __ballot(lb != 0);
