@@ -109,10 +109,9 @@ __device__ __forceinline__ RowMinMax row_minmax(uint32_t x0, uint32_t x1, uint32
 
 template <int T>
 struct TileSmem {
-    static constexpr int P = T + 8;   // LDS row pitch in bytes
+    static constexpr int P = T + 16;  // LDS row pitch in bytes (strips of 4 rows hit disjoint banks)
     alignas(16) uint8_t img[(T + 2) * P];
     alignas(16) uint32_t lab[T * T];  // labels; at roots also evidence and fragment id
-    uint8_t lut[4 * T];               // write-back: fragment -> 0x80 | new mask bits, or 0
     uint8_t eqr[T];                   // cross-seam equality bits, right column
     uint8_t eqb[T];                   // cross-seam equality bits, bottom row
     uint32_t count;
@@ -122,13 +121,45 @@ struct TileSmem {
     uint32_t open;                    // the tile holds a fragment that may still change
 };
 
+// Thread -> quad mapping.  With fewer threads than quads a thread owns a
+// vertical strip of PER quads (rows r0 .. r0 + PER - 1 of one quad column), so
+// the strip's rows are loaded and reduced once and shared by the quads above
+// and below them.
 template <int T, int NT>
 struct Quads {
     static constexpr int D = T / 4;                  // quads per row
     static constexpr int Q = T * T / 4;              // quads per tile
     static constexpr int PER = (Q + NT - 1) / NT;    // quads per thread
     static constexpr int PPER = (4 * T + NT - 1) / NT;  // perimeter slots per thread
+    static_assert(PER == 1 || Q % NT == 0, "strips need NT | Q");
+    // Row and first column of the calling thread's quad k; false if the
+    // thread has no quad k.
+    __device__ static __forceinline__ bool at(int k, int& ly, int& lx0) {
+        if constexpr (PER == 1) {
+            const int q = threadIdx.x;
+            ly = q / D;
+            lx0 = 4 * (q % D);
+            return Q % NT == 0 || q < Q;
+        } else {
+            ly = (int)(threadIdx.x / D) * PER + k;
+            lx0 = 4 * (int)(threadIdx.x % D);
+            return true;
+        }
+    }
 };
+
+// Per-quad evidence in byte form (bit b of byte i = flag of pixel i), the
+// form the classification produces and the mask stores consume.
+constexpr uint32_t B_G = 0x80808080u;     // a strictly greater 8-neighbour exists
+constexpr uint32_t B_S = 0x40404040u;     // a strictly smaller 8-neighbour exists
+constexpr uint32_t B_SGL = 0x20202020u;   // singleton
+constexpr uint32_t B_VAL = 0x10101010u;   // pixel inside the image part of the tile
+// 0x80-per-byte flags -> 0xFF bytes.
+__device__ __forceinline__ uint32_t bytes80(uint32_t x) { return (x - (x >> 7)) | x; }
+// Two 0x80-per-byte flag words -> one byte: a's pixels in bits 0-3, b's in 4-7.
+__device__ __forceinline__ uint32_t movemask_pair(uint32_t a, uint32_t b) {
+    return ((((a >> 7) | (b >> 3)) & 0x11111111u) * 0x01020408u) >> 24;
+}
 
 __device__ __forceinline__ uint32_t lds32(const uint8_t* p) { return *reinterpret_cast<const uint32_t*>(p); }
 
@@ -202,84 +233,124 @@ __device__ __forceinline__ void tile_load(TileSmem<T>& sm, const ImgDev& im, int
     }
 }
 
-// First pass over the thread's quads: pixel evidence, Komura init (link to the
-// first equal predecessor in the order up, up-left, up-right, left), the edge
-// bits of the union pass and singleton detection (no equal value among the
-// eight LDS neighbours, halo included -- conservative at the image border,
-// where a replicated halo value may only make a pixel non-singleton).
-// Returns whether every pixel equals v0.
-template <int T, int NT>
-__device__ __forceinline__ bool quad_init(TileSmem<T>& sm, int tw, int th, uint32_t (&st)[Quads<T, NT>::PER],
-                                          uint32_t v0) {
-    constexpr int P = TileSmem<T>::P, D = Quads<T, NT>::D, Q = Quads<T, NT>::Q;
-    bool all_eq = true;
-#pragma unroll
-    for (int k = 0; k < Quads<T, NT>::PER; ++k) {
-        st[k] = 0;
-        const int q = threadIdx.x + k * NT;
-        if (Q % NT != 0 && q >= Q) continue;
-        const int ly = q / D, lx0 = 4 * (q % D);
-        if (ly >= th || lx0 >= tw) continue;
-        const int base = (ly + 1) * P + 4 + lx0;
-        const uint32_t a0 = lds32(&sm.img[base - P - 4]), a1 = lds32(&sm.img[base - P]),
-                       a2 = lds32(&sm.img[base - P + 4]);
-        const uint32_t b0 = lds32(&sm.img[base - 4]), b1 = lds32(&sm.img[base]), b2 = lds32(&sm.img[base + 4]);
-        const uint32_t c0 = lds32(&sm.img[base + P - 4]), c1 = lds32(&sm.img[base + P]),
-                       c2 = lds32(&sm.img[base + P + 4]);
-        // Neighbour words: byte i is the neighbour of pixel i in that direction.
-        const uint32_t V = b1;
-        const uint32_t nU = ne_top(a1, V), nUL = ne_top(__builtin_amdgcn_alignbyte(a1, a0, 3), V),
-                       nUR = ne_top(__builtin_amdgcn_alignbyte(a2, a1, 1), V);
-        const uint32_t nL = ne_top(__builtin_amdgcn_alignbyte(b1, b0, 3), V),
-                       nR = ne_top(__builtin_amdgcn_alignbyte(b2, b1, 1), V);
-        const uint32_t nD = ne_top(c1, V), nDL = ne_top(__builtin_amdgcn_alignbyte(c1, c0, 3), V),
-                       nDR = ne_top(__builtin_amdgcn_alignbyte(c2, c1, 1), V);
-        // 3x3 max / min (self included: it never makes a neighbour greater or
-        // smaller) on even / odd 16-bit lanes.
-        const RowMinMax ru = row_minmax(a0, a1, a2), rc = row_minmax(b0, b1, b2), rd = row_minmax(c0, c1, c2);
-        const uint32_t ge = pk_subsat(pk_max(pk_max(ru.maxe, rc.maxe), rd.maxe), rc.mid_e);
-        const uint32_t go = pk_subsat(pk_max(pk_max(ru.maxo, rc.maxo), rd.maxo), rc.mid_o);
-        const uint32_t se = pk_subsat(rc.mid_e, pk_min(pk_min(ru.mine, rc.mine), rd.mine));
-        const uint32_t so = pk_subsat(rc.mid_o, pk_min(pk_min(ru.mino, rc.mino), rd.mino));
-        const uint32_t g4 = movemask4((nz_top16(ge) >> 8) | nz_top16(go));
-        const uint32_t s4 = movemask4((nz_top16(se) >> 8) | nz_top16(so));
-        const uint32_t val4 = tw - lx0 >= 4 ? 0xFu : ((1u << (tw - lx0)) - 1u);
-        const uint32_t lv4 = lx0 > 0 ? 0xFu : 0xEu;   // pixel has a left neighbour in the tile
-        // pixel i has a right neighbour in the tile iff lx0 + i + 1 < tw
-        const uint32_t rv4 = tw - lx0 - 1 >= 4 ? 0xFu : ((1u << (tw - lx0 - 1)) - 1u);
-        // pixels past tw hold no image data: every edge bit is masked to valid pixels
-        const uint32_t up4 = ly > 0 ? val4 : 0u;
-        const uint32_t any_eq4 = movemask4(~(nU & nUL & nUR & nL & nR & nD & nDL & nDR) & 0x80808080u);
-        const uint32_t eu4 = movemask4(nU ^ 0x80808080u) & up4;
-        const uint32_t eul4 = movemask4(nUL ^ 0x80808080u) & up4 & lv4;
-        const uint32_t eur4 = movemask4(nUR ^ 0x80808080u) & up4 & rv4;
-        const uint32_t el4 = movemask4(nL ^ 0x80808080u) & lv4 & val4;
-        all_eq = all_eq && (movemask4(ne_top(V, v0 * 0x01010101u)) & val4) == 0;
-        const uint32_t sgl4 = ~any_eq4 & val4;
-        const uint32_t s = g4 << Q_G | s4 << Q_S | el4 << Q_EL | eu4 << Q_EU | eul4 << Q_EUL | eur4 << Q_EUR |
-                           sgl4 << Q_SGL | val4 << Q_VAL;
-        st[k] = s;
-        const uint32_t nsg = val4 & ~sgl4;
-        if (!nsg) continue;
+// One image row around a quad: the quad's dword, its left- and right-shifted
+// neighbour words (byte i = neighbour of pixel i) and the row's 3-wide
+// extrema.  Row ly = -1 and ly = T are the halo rows.
+struct RowW {
+    uint32_t w, l, r;
+    RowMinMax m;
+};
+template <int T>
+__device__ __forceinline__ RowW row_at(const TileSmem<T>& sm, int ly, int lx0) {
+    constexpr int P = TileSmem<T>::P;
+    const int base = (ly + 1) * P + 4 + lx0;
+    const uint32_t x0 = lds32(&sm.img[base - 4]), x1 = lds32(&sm.img[base]), x2 = lds32(&sm.img[base + 4]);
+    RowW r;
+    r.w = x1;
+    r.l = __builtin_amdgcn_alignbyte(x1, x0, 3);
+    r.r = __builtin_amdgcn_alignbyte(x2, x1, 1);
+    r.m = row_minmax(x0, x1, x2);
+    return r;
+}
+
+// 0x80 in bytes 0 .. n-1 (n >= 4: all four, n <= 0: none).
+__device__ __forceinline__ uint32_t first_bytes80(int n) {
+    return n >= 4 ? 0x80808080u : (n <= 0 ? 0u : (0x80808080u >> (8 * (4 - n))));
+}
+
+// Classifies one quad from its three rows: byte-form evidence (returned), and
+// for a quad holding a non-singleton pixel also the 4-bit edge word of the
+// labelling passes (else 0) and its Komura labels.  Singletons (no equal value among
+// the eight LDS neighbours, halo included -- conservative at the image border,
+// where a replicated halo value may only make a pixel non-singleton) take no
+// label.  Pixels past tw or th are masked out of every flag.
+template <int T>
+__device__ __forceinline__ uint32_t classify_quad(TileSmem<T>& sm, const RowW& a, const RowW& b, const RowW& c,
+                                                  int ly, int lx0, int tw, int th, uint32_t v4, bool& all_eq,
+                                                  uint32_t& st) {
+    const uint32_t V = b.w;
+    const uint32_t nU = ne_top(a.w, V), nUL = ne_top(a.l, V), nUR = ne_top(a.r, V);
+    const uint32_t nL = ne_top(b.l, V), nR = ne_top(b.r, V);
+    const uint32_t nD = ne_top(c.w, V), nDL = ne_top(c.l, V), nDR = ne_top(c.r, V);
+    // 3x3 max / min (self included: it never makes a neighbour greater or
+    // smaller) on even / odd 16-bit lanes
+    const uint32_t ge = pk_subsat(pk_max(pk_max(a.m.maxe, b.m.maxe), c.m.maxe), b.m.mid_e);
+    const uint32_t go = pk_subsat(pk_max(pk_max(a.m.maxo, b.m.maxo), c.m.maxo), b.m.mid_o);
+    const uint32_t se = pk_subsat(b.m.mid_e, pk_min(pk_min(a.m.mine, b.m.mine), c.m.mine));
+    const uint32_t so = pk_subsat(b.m.mid_o, pk_min(pk_min(a.m.mino, b.m.mino), c.m.mino));
+    const uint32_t g80 = (nz_top16(ge) >> 8) | nz_top16(go);
+    const uint32_t s80 = (nz_top16(se) >> 8) | nz_top16(so);
+    const uint32_t val80 = (ly < th) ? first_bytes80(tw - lx0) : 0u;
+    const uint32_t neq = nU & nUL & nUR & nL & nR & nD & nDL & nDR;   // 0x80: no equal neighbour
+    const uint32_t sgl80 = neq & val80;
+    all_eq = all_eq && (ne_top(V, v4) & val80) == 0;
+    st = 0;
+    if (val80 & ~neq) {
+        const uint32_t lv80 = lx0 > 0 ? 0x80808080u : 0x80808000u;   // left neighbour in the tile
+        const uint32_t rv80 = first_bytes80(tw - lx0 - 1);              // right neighbour in the tile
+        const uint32_t up80 = ly > 0 ? val80 : 0u;
+        const uint32_t eu80 = ~nU & up80, eul80 = ~nUL & up80 & lv80, eur80 = ~nUR & up80 & rv80;
+        const uint32_t el80 = ~nL & lv80 & val80;
+        const uint32_t s = movemask_pair(g80, s80) << Q_G | movemask_pair(el80, eu80) << Q_EL |
+                           movemask_pair(eul80, eur80) << Q_EUL | movemask_pair(sgl80, val80) << Q_SGL;
+        st = s;
         // Labels of the quad in one 16-byte store: a head links to its first
         // contact above (else to the pixel left of the quad, else to itself),
         // every other run pixel to its head.  Singletons get a dummy entry
         // that nothing reads.  Every link points at a smaller index.
         const uint32_t p0 = ly * T + lx0;
-        const uint32_t hd = nsg & ~(el4 & 0xEu);
+        const uint32_t hd = q_heads(s);
+        const uint32_t el4 = (s >> Q_EL) & 0xFu;
         uint32_t lab4[4];
         uint32_t head = p0;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             if ((hd >> i) & 1u) {
                 head = p0 + i;
-                const uint32_t a = q_above(s, q_run(s, i));
-                lab4[i] = a ? (p0 - T + __builtin_ctz(a) - 1u) : ((i == 0 && (el4 & 1u)) ? p0 - 1u : p0 + i);
+                const uint32_t ab = q_above(s, q_run(s, i));
+                lab4[i] = ab ? (p0 - T + __builtin_ctz(ab) - 1u) : ((i == 0 && (el4 & 1u)) ? p0 - 1u : p0 + i);
             } else {
                 lab4[i] = head;   // run pixel (or an unread dummy)
             }
         }
         *reinterpret_cast<uint4*>(&sm.lab[p0]) = make_uint4(lab4[0], lab4[1], lab4[2], lab4[3]);
+    }
+    return g80 | (s80 >> 1) | (sgl80 >> 2) | (val80 >> 3);
+}
+
+// First pass over the thread's quads (a vertical strip): evidence, singleton
+// detection, the edge bits of the union passes and the Komura links (each
+// head links to an equal pixel above, else left).  sb[k] gets the byte-form
+// evidence, st[k] the edge word (0 for a quad of singletons).  Returns
+// whether every pixel equals v0.
+template <int T, int NT>
+__device__ __forceinline__ bool quad_init(TileSmem<T>& sm, int tw, int th, uint32_t (&st)[Quads<T, NT>::PER],
+                                          uint32_t (&sb)[Quads<T, NT>::PER], uint32_t v0) {
+    constexpr int PER = Quads<T, NT>::PER;
+    const uint32_t v4 = v0 * 0x01010101u;
+    bool all_eq = true;
+    int ly, lx0;
+    const bool has = Quads<T, NT>::at(0, ly, lx0);
+    if constexpr (PER == 1) {
+        st[0] = 0;
+        sb[0] = 0;
+        if (!has || ly >= th || lx0 >= tw) return true;
+        const RowW a = row_at<T>(sm, ly - 1, lx0), b = row_at<T>(sm, ly, lx0), c = row_at<T>(sm, ly + 1, lx0);
+        sb[0] = classify_quad<T>(sm, a, b, c, ly, lx0, tw, th, v4, all_eq, st[0]);
+    } else {
+        const int r0 = ly;
+        const bool col_ok = lx0 < tw;
+        RowW a = row_at<T>(sm, r0 - 1, lx0), b = row_at<T>(sm, r0, lx0);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const RowW c = row_at<T>(sm, r0 + k + 1, lx0);
+            st[k] = 0;
+            sb[k] = 0;
+            if (col_ok && r0 + k < th)
+                sb[k] = classify_quad<T>(sm, a, b, c, r0 + k, lx0, tw, th, v4, all_eq, st[k]);
+            a = b;
+            b = c;
+        }
     }
     return all_eq;
 }
@@ -328,17 +399,21 @@ __device__ __forceinline__ uint32_t perimeter_pass(TileSmem<T>& sm, const ImgDev
     return seam_bits;
 }
 
-// Labels of the non-singleton pixels: compress, unite the left and up-right
-// edges (min-root CAS), then a read-only final compression that also ORs each
-// pixel's evidence, and the seam evidence of the perimeter slots, into the
-// roots.  Assumes a barrier after quad_init; ends with a barrier.
+// Labels of the non-singleton pixels: compression of the run heads, unions of
+// the edges the Komura links do not cover (further contact segments above a
+// run: a segment is one zone, its pixels being equal and horizontally
+// adjacent; and the left contact of a run that starts the quad), then a
+// read-only final compression that also ORs each run's evidence, and the seam
+// evidence of the perimeter slots, into the roots.  Assumes a barrier after
+// quad_init; ends with one.
 template <int T, int NT>
 __device__ __forceinline__ void quad_label(TileSmem<T>& sm, const uint32_t (&st)[Quads<T, NT>::PER], int tw,
                                            int th, uint32_t seam_bits) {
-    constexpr int D = Quads<T, NT>::D;
 #pragma unroll
     for (int k = 0; k < Quads<T, NT>::PER; ++k) {
-        const int q = threadIdx.x + k * NT, p0 = (q / D) * T + 4 * (q % D);
+        int ly, lx0;
+        Quads<T, NT>::at(k, ly, lx0);
+        const int p0 = ly * T + lx0;
         uint32_t hd = q_heads(st[k]);
         while (hd) {
             const int i = __builtin_ctz(hd);
@@ -347,18 +422,17 @@ __device__ __forceinline__ void quad_label(TileSmem<T>& sm, const uint32_t (&st)
         }
     }
     __syncthreads();
-    // Unions of the edges the head links do not cover: the further contact
-    // segments above a run (a segment is one zone: its pixels are equal and
-    // horizontally adjacent) and the left contact of a run that starts the quad.
 #pragma unroll
     for (int k = 0; k < Quads<T, NT>::PER; ++k) {
-        const int q = threadIdx.x + k * NT, ly = q / D, p0 = ly * T + 4 * (q % D);
+        int ly, lx0;
+        Quads<T, NT>::at(k, ly, lx0);
+        const int p0 = ly * T + lx0;
         uint32_t hd = q_heads(st[k]);
         while (hd) {
             const int i = __builtin_ctz(hd);
             hd &= hd - 1u;
-            const uint32_t a = q_above(st[k], q_run(st[k], i));
-            uint32_t seg = a & ~(a << 1);
+            const uint32_t ab = q_above(st[k], q_run(st[k], i));
+            uint32_t seg = ab & ~(ab << 1);
             if (!seg) continue;   // linked left or alone: nothing more to unite
             seg &= seg - 1u;      // the first segment is the head's link
             while (seg) {
@@ -372,7 +446,9 @@ __device__ __forceinline__ void quad_label(TileSmem<T>& sm, const uint32_t (&st)
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < Quads<T, NT>::PER; ++k) {
-        const int q = threadIdx.x + k * NT, p0 = (q / D) * T + 4 * (q % D);
+        int ly, lx0;
+        Quads<T, NT>::at(k, ly, lx0);
+        const int p0 = ly * T + lx0;
         uint32_t hd = q_heads(st[k]);
         while (hd) {
             const int i = __builtin_ctz(hd);
@@ -411,6 +487,17 @@ __device__ __forceinline__ void store_quad(uint8_t* dst, uint32_t packed, uint32
     }
 }
 
+// Same with the selection as 0xFF bytes.
+__device__ __forceinline__ void store_bytes(uint8_t* dst, uint32_t packed, uint32_t selb, bool aligned) {
+    if (selb == 0xFFFFFFFFu && aligned) {
+        *reinterpret_cast<uint32_t*>(dst) = packed;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if ((selb >> (8 * i)) & 1u) dst[i] = (uint8_t)(packed >> (8 * i));
+    }
+}
+
 // Perimeter records of the thread's seam slots: fragment j in every perimeter
 // array the pixel belongs to.
 template <int T, int NT>
@@ -440,7 +527,7 @@ __device__ __forceinline__ void write_perimeter_records(TileSmem<T>& sm, uint8_t
 // Tile stage for one tile (ss_device.hpp, step 1).
 template <int T, int NT>
 __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, int64_t gtile, const Ws& ws) {
-    constexpr int S = 4 * T - 4, D = Quads<T, NT>::D, PER = Quads<T, NT>::PER;
+    constexpr int S = 4 * T - 4, PER = Quads<T, NT>::PER;
     const int64_t lt = gtile - im.tile_begin;
     const int tx = (int)(lt % im.tiles_x), ty = (int)(lt / im.tiles_x);
     const int x0 = tx * T, y0 = ty * T;
@@ -454,8 +541,8 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
     tile_load<T, NT>(sm, im, x0, y0, tw, th);
     __syncthreads();
     const uint32_t v0 = sm.img[TileSmem<T>::P + 4];
-    uint32_t st[PER];
-    const bool mine_eq = quad_init<T, NT>(sm, tw, th, st, v0);
+    uint32_t st[PER], sb[PER];
+    const bool mine_eq = quad_init<T, NT>(sm, tw, th, st, sb, v0);
     const uint32_t seam_bits = perimeter_pass<T, NT>(sm, im, x0, y0, tw, th);
     const bool uniform = __syncthreads_and(mine_eq) != 0;
     const bool al_out = (((uintptr_t)im.mx | (uintptr_t)im.mn | (uintptr_t)im.opitch) & 3u) == 0;
@@ -466,9 +553,8 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
         uint32_t f = seam_bits ? F_SEAM : 0u;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            const uint32_t val4 = (st[k] >> Q_VAL) & 0xFu;
-            f |= ((st[k] >> Q_G) & val4) ? F_G : 0u;
-            f |= ((st[k] >> Q_S) & val4) ? F_S : 0u;
+            f |= (sb[k] & (sb[k] << 3) & B_G) ? F_G : 0u;          // evidence of a valid pixel
+            f |= ((sb[k] << 1) & (sb[k] << 3) & B_G) ? F_S : 0u;
         }
         if (f) atomicOr(&sm.zflags, f);
         __syncthreads();
@@ -480,12 +566,13 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
         const uint32_t mxb = (zf & F_G) ? 0u : 0xFFFFFFFFu, mnb = (zf & F_S) ? 0u : 0xFFFFFFFFu;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            const uint32_t sel = (st[k] >> Q_VAL) & 0xFu;
-            if (!sel) continue;
-            const int q = threadIdx.x + k * NT, ly = q / D, lx0 = 4 * (q % D);
+            if (!(sb[k] & B_VAL)) continue;
+            int ly, lx0;
+            Quads<T, NT>::at(k, ly, lx0);
+            const uint32_t selb = bytes80((sb[k] << 3) & B_G);
             const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
-            store_quad(im.mx + o, mxb, sel, al_out);
-            store_quad(im.mn + o, mnb, sel, al_out);
+            store_bytes(im.mx + o, mxb, selb, al_out);
+            store_bytes(im.mn + o, mnb, selb, al_out);
         }
         write_perimeter_records<T, NT>(sm, rec, tw, th, seam_bits, true);
     } else {
@@ -493,7 +580,9 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
         // Number the seam zones (their roots) and fill their fragment slots.
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            const int q = threadIdx.x + k * NT, p0 = (q / D) * T + 4 * (q % D);
+            int ly, lx0;
+            Quads<T, NT>::at(k, ly, lx0);
+            const int p0 = ly * T + lx0;
             uint32_t hd = q_heads(st[k]);
             while (hd) {
                 const int i = __builtin_ctz(hd);
@@ -520,12 +609,13 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
         uint8_t* plane = ws.rec8 + gtile * (int64_t)(T * T);
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            const uint32_t sel = (st[k] >> Q_VAL) & 0xFu;
-            if (!sel) continue;
-            const int q = threadIdx.x + k * NT, ly = q / D, lx0 = 4 * (q % D);
+            const uint32_t b = sb[k];
+            if (!(b & B_VAL)) continue;
+            int ly, lx0;
+            Quads<T, NT>::at(k, ly, lx0);
+            const uint32_t selb = bytes80((b << 3) & B_G);
             // singletons straight from the evidence bits, the others from their root
-            const uint32_t sgl = bytes4((st[k] >> Q_SGL) & 0xFu);
-            uint32_t mxw = ~bytes4((st[k] >> Q_G) & 0xFu) & sgl, mnw = ~bytes4((st[k] >> Q_S) & 0xFu) & sgl;
+            uint32_t mxw = bytes80((b << 2) & ~b & B_G), mnw = bytes80((b << 2) & ~(b << 1) & B_G);
             uint32_t rcw = 0xFFFFFFFFu;
             uint32_t hd = q_heads(st[k]);
             while (hd) {
@@ -540,8 +630,8 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
                 if (!(zf & F_S)) mnw |= rb;
             }
             const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
-            store_quad(im.mx + o, mxw, sel, al_out);
-            store_quad(im.mn + o, mnw, sel, al_out);
+            store_bytes(im.mx + o, mxw, selb, al_out);
+            store_bytes(im.mn + o, mnw, selb, al_out);
             if (open) *reinterpret_cast<uint32_t*>(plane + ly * T + lx0) = rcw;
         }
         write_perimeter_records<T, NT>(sm, rec, tw, th, seam_bits, false);
