@@ -207,29 +207,63 @@ __device__ __forceinline__ void lunion_h(uint32_t* lab, uint32_t a, uint32_t b) 
 // belongs to the pixel itself or to one of its genuine 8-neighbours, so the
 // 3x3 greater / smaller tests equal the "absent neighbour" rule exactly
 // (equality across the image border is excluded by the perimeter pass).
+// All of a thread's global loads are issued before the first LDS store, so a
+// tile costs one memory round trip rather than one per loop iteration.
 template <int T, int NT>
 __device__ __forceinline__ void tile_load(TileSmem<T>& sm, const ImgDev& im, int x0, int y0, int tw, int th) {
     constexpr int P = TileSmem<T>::P, D = T / 4;
+    constexpr int ITEMS = ((T + 2) * D + NT - 1) / NT;   // dwords per thread
+    constexpr int HALO = (2 * (T + 2) + NT - 1) / NT;    // halo-column bytes per thread
     const bool al4 = ((reinterpret_cast<uintptr_t>(im.img) | (uintptr_t)im.pitch) & 3u) == 0;
-    for (int it = threadIdx.x; it < (th + 2) * D; it += NT) {
-        const int r = it / D - 1, c = 4 * (it % D);
-        if (c >= tw) continue;
+    auto row_of = [&](int r) -> const uint8_t* {
         int gy = y0 + r;
         gy = gy < 0 ? 0 : (gy >= im.h ? im.h - 1 : gy);
-        const uint8_t* src = im.img + (int64_t)gy * im.pitch + x0 + c;
-        uint8_t* dst = &sm.img[(r + 1) * P + 4 + c];
+        return im.img + (int64_t)gy * im.pitch + x0;
+    };
+    uint32_t v[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const int it = threadIdx.x + i * NT;
+        const int r = it / D - 1, c = 4 * (it % D);
+        v[i] = 0;
+        if (it >= (th + 2) * D || c >= tw) continue;
+        const uint8_t* src = row_of(r) + c;
         if (al4 && c + 3 < tw) {
-            *reinterpret_cast<uint32_t*>(dst) = *reinterpret_cast<const uint32_t*>(src);
+            v[i] = *reinterpret_cast<const uint32_t*>(src);
         } else {
-            for (int b = 0; b < 4 && c + b < tw; ++b) dst[b] = src[b];
+            for (int b = 0; b < 4 && c + b < tw; ++b) v[i] |= (uint32_t)src[b] << (8 * b);
         }
     }
-    for (int it = threadIdx.x; it < 2 * (th + 2); it += NT) {
+    uint8_t hv[HALO];
+#pragma unroll
+    for (int i = 0; i < HALO; ++i) {
+        const int it = threadIdx.x + i * NT;
+        hv[i] = 0;
+        if (it >= 2 * (th + 2)) continue;
         const int r = it / 2 - 1, col = (it & 1) ? tw : -1;
-        int gy = y0 + r, gx = x0 + col;
-        gy = gy < 0 ? 0 : (gy >= im.h ? im.h - 1 : gy);
+        int gx = x0 + col;
         gx = gx < 0 ? 0 : (gx >= im.w ? im.w - 1 : gx);
-        sm.img[(r + 1) * P + 4 + col] = im.img[(int64_t)gy * im.pitch + gx];
+        hv[i] = row_of(r)[gx - x0];
+    }
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const int it = threadIdx.x + i * NT;
+        const int r = it / D - 1, c = 4 * (it % D);
+        if (it >= (th + 2) * D || c >= tw) continue;
+        uint8_t* dst = &sm.img[(r + 1) * P + 4 + c];
+        if (c + 3 < tw) {
+            *reinterpret_cast<uint32_t*>(dst) = v[i];
+        } else {
+            // bytes past tw stay untouched: column tw is the halo column
+            for (int b = 0; c + b < tw; ++b) dst[b] = (uint8_t)(v[i] >> (8 * b));
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < HALO; ++i) {
+        const int it = threadIdx.x + i * NT;
+        if (it >= 2 * (th + 2)) continue;
+        const int r = it / 2 - 1, col = (it & 1) ? tw : -1;
+        sm.img[(r + 1) * P + 4 + col] = hv[i];
     }
 }
 
@@ -440,7 +474,10 @@ __device__ __forceinline__ void quad_label(TileSmem<T>& sm, const uint32_t (&st)
                 seg &= seg - 1u;
                 lunion_h(sm.lab, p0 + i, p0 - T + j - 1);
             }
-            if (i == 0 && ((st[k] >> Q_EL) & 1u)) lunion_h(sm.lab, p0, p0 - 1);
+            // The left pixel needs a union only if the run touches neither of
+            // its two upper neighbours: such a contact is one of the left
+            // pixel's own contacts, to which both runs are already joined.
+            if (i == 0 && ((st[k] >> Q_EL) & 1u) && !(ab & 3u)) lunion_h(sm.lab, p0, p0 - 1);
         }
     }
     __syncthreads();
