@@ -143,6 +143,24 @@ def constant(h, w, value=77, device="cpu"):
     return torch.full((h, w), value, dtype=torch.uint8, device=device)
 
 
+def seam_comb(h, w, tile, device="cpu"):
+    """Every perimeter pixel of every tile its own seam fragment (up to 4T-4
+    per tile).  Rows next to a horizontal seam cycle through 10, 20, 30 along
+    x, so a pixel equals only the pixel straight across the seam; columns next
+    to a vertical seam cycle through 40, 50, 60 along y; the rows and columns
+    one step inside (200, 210) keep each perimeter pixel apart from the
+    tile's interior (128)."""
+    y = torch.arange(h, device=device)[:, None]
+    x = torch.arange(w, device=device)[None, :]
+    ry, rx = y % tile, x % tile
+    v = torch.full((h, w), 128, dtype=torch.int64, device=device)
+    v = torch.where((rx == 1) | (rx == tile - 2), torch.full_like(v, 210), v)
+    v = torch.where((rx == 0) | (rx == tile - 1), (40 + 10 * (y % 3)).expand(h, w), v)
+    v = torch.where((ry == 1) | (ry == tile - 2), torch.full_like(v, 200), v)
+    v = torch.where((ry == 0) | (ry == tile - 1), (10 + 10 * (x % 3)).expand(h, w), v)
+    return v.to(torch.uint8)
+
+
 _TEMPLATES = {}
 
 
@@ -205,6 +223,7 @@ def adversarial_images(side=4096, tile=32, seed=0, device="cpu"):
     yield "serpentine_h", serpentine(side, side, seed, False, device)
     yield "serpentine_v", serpentine(side, side, seed, True, device)
     yield "checker2", checker(side, side, device=device)
+    yield "seam_comb", seam_comb(side, side, tile, device=device)
     yield "constant", constant(side, side, device=device)
 
 

@@ -7,7 +7,7 @@ import numpy as np
 import torch
 from scipy import ndimage
 
-from sscommon import oracle_bruteforce, oracle_quantize
+from sscommon import oracle_bruteforce, oracle_quantize, oracle_sparse_seam
 from ssx import ssgen
 
 CONFIG1_SHA256 = "f74dafd397f7006f"  # first 16 hex digits, 512x512 sigma 2 q32 seed 0
@@ -93,3 +93,14 @@ def test_mixed_sizes_range():
     hs = [h for h, _ in sizes]
     ws = [w for _, w in sizes]
     assert min(hs) >= 256 and max(hs) <= 1024 and min(ws) >= 256 and max(ws) <= 1024
+
+
+def test_seam_comb_fills_fragment_capacity():
+    for tile in (16, 32):
+        img = ssgen.seam_comb(256, 192, tile).numpy()
+        _, _, counts = oracle_sparse_seam(img, tile)
+        # interior tiles: every perimeter pixel but the two corners a row/column
+        # pattern shares with its neighbour is a fragment of its own
+        assert counts[1:-1, 1:-1].min() >= 4 * tile - 6
+        assert counts.max() <= 4 * tile - 4
+        assert np.array_equal(*oracle_bruteforce(img)) is False

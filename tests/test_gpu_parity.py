@@ -129,6 +129,21 @@ def test_adversarial_4096(dev):
                 assert not (mx[h == 254] == 255).any()
 
 
+def test_seam_comb_fragment_capacity(dev):
+    """Every perimeter pixel a fragment: tiles near 4T-4 fragments, and at
+    T = 64 merge blocks with more fragments than the block merge keeps in LDS
+    (its dense-array path)."""
+    for tile in ss.TILES:
+        img = ssgen.seam_comb(1024, 1024, tile).numpy()
+        say("seam_comb 1024 tile=%d" % tile)
+        emax, emin, ecounts = oracle_sparse_seam(img, tile)
+        mx, mn, ws, st = run(img, tile, 256)
+        assert np.array_equal(mx, emax) and np.array_equal(mn, emin), tile
+        counts = ss.tile_counts(ws, *ecounts.shape)
+        assert np.array_equal(counts, ecounts), tile
+        assert counts.max() >= 4 * tile - 6
+
+
 def test_large_images_vs_global_uf(dev):
     for side in (4096, 8192):
         for name, img in ssgen.sweep_images(side, 0, device="cuda", sigmas=(4,), levels=(2, 16)):
