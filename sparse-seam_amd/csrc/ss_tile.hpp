@@ -746,25 +746,37 @@ __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, i
         if (ly >= th || c0 >= tw) continue;
         const uint4 rv = *reinterpret_cast<const uint4*>(plane + ly * T + c0);
         const uint32_t rw[4] = {rv.x, rv.y, rv.z, rv.w};
+        uint32_t mxw[4], mnw[4], sel[4];
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
             const int lx0 = c0 + 4 * d;
-            if (lx0 >= tw) break;
-            uint32_t mxw = 0, mnw = 0, sel = 0;
+            mxw[d] = 0;
+            mnw[d] = 0;
+            sel[d] = 0;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const uint32_t r = (rw[d] >> (8 * i)) & 0xFFu;
                 if (lx0 + i >= tw || r == REC_NONE) continue;
                 const uint32_t v = lut[r];
                 if (!(v & 0x80u)) continue;
-                sel |= 1u << i;
-                mxw |= ((v & 1u) ? 0xFFu : 0u) << (8 * i);
-                mnw |= ((v & 2u) ? 0xFFu : 0u) << (8 * i);
+                sel[d] |= 1u << i;
+                mxw[d] |= ((v & 1u) ? 0xFFu : 0u) << (8 * i);
+                mnw[d] |= ((v & 2u) ? 0xFFu : 0u) << (8 * i);
             }
-            if (!sel) continue;
-            const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
-            store_quad(im.mx + o, mxw, sel, al_out);
-            store_quad(im.mn + o, mnw, sel, al_out);
+        }
+        const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + c0;
+        const bool al16 = (((uintptr_t)im.mx | (uintptr_t)im.mn | (uintptr_t)im.opitch) & 15u) == 0;
+        if (al16 && (sel[0] & sel[1] & sel[2] & sel[3]) == 0xFu) {
+            // a whole 16-byte chunk: one store per mask, 16 rows per wave store
+            *reinterpret_cast<uint4*>(im.mx + o) = make_uint4(mxw[0], mxw[1], mxw[2], mxw[3]);
+            *reinterpret_cast<uint4*>(im.mn + o) = make_uint4(mnw[0], mnw[1], mnw[2], mnw[3]);
+        } else {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                if (!sel[d]) continue;
+                store_quad(im.mx + o + 4 * d, mxw[d], sel[d], al_out);
+                store_quad(im.mn + o + 4 * d, mnw[d], sel[d], al_out);
+            }
         }
     }
     return true;
