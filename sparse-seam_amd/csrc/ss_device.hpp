@@ -44,7 +44,7 @@ constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_CHUNK = SCAN_BLOCK * SCAN_ITEMS;  // tiles per scan block
 constexpr int NUM_XCD = 8;
 constexpr int MERGE_BT = 8;        // tiles per side of a block-local seam merge
-constexpr int MERGE_THREADS = 256;
+constexpr int MERGE_THREADS = 1024;
 
 // One image of a launch set.  tile_begin is the global id of its first tile,
 // block_begin that of its first merge block (MERGE_BT x MERGE_BT tiles).
