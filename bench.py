@@ -76,21 +76,25 @@ def batched_leg(n, dev, ss, ssgen, torch, reps=3):
     log("batch: generated %d images (%.2f Gpx) in %.1fs" % (n, px / 1e9, time.time() - t0))
     step = max(1, n // 64)
     sample = list(range(0, n, step))
-    (tile, threads), _ = ss.autotune(ss.descs([imgs[i] for i in sample], [outs[i] for i in sample]), reps=2)
-    log("batch: autotuned tile=%d threads=%d" % (tile, threads))
+    sample_descs = ss.descs([imgs[i] for i in sample], [outs[i] for i in sample])
+    # each path with the configuration its own autotuner picks on the sample:
+    # per-image launches by the per-image rule, the batch by batched launches
+    (ptile, pthreads), _ = ss.autotune(sample_descs, reps=2)
+    (tile, threads), _ = ss.autotune(sample_descs, reps=2, batch=True)
+    log("batch: autotuned per-image tile=%d threads=%d, batched tile=%d threads=%d" % (ptile, pthreads, tile, threads))
     stream = torch.cuda.current_stream()
     ev = lambda: torch.cuda.Event(enable_timing=True)
     # per-image launches of the plain path
-    ws = ss.new_workspace(max(ss.workspace_size(w, h, tile, threads) for h, w in sizes), dev)
+    ws = ss.new_workspace(max(ss.workspace_size(w, h, ptile, pthreads) for h, w in sizes), dev)
     for img, o in zip(imgs, outs):
-        ss.extrema(img, tile, threads, ws, o)
+        ss.extrema(img, ptile, pthreads, ws, o)
     torch.cuda.synchronize()
     plain_ms = []
     for _ in range(reps):
         a, b = ev(), ev()
         a.record(stream)
         for img, o in zip(imgs, outs):
-            ss.extrema(img, tile, threads, ws, o)
+            ss.extrema(img, ptile, pthreads, ws, o)
         b.record(stream)
         torch.cuda.synchronize()
         plain_ms.append(a.elapsed_time(b))
@@ -120,7 +124,7 @@ def batched_leg(n, dev, ss, ssgen, torch, reps=3):
     return {
         "workload": "%d images, W,H iid U[256,1024], content iid / Gaussian sigma 3 q16 / EDT of 20..300 "
                     "disks (1:1:1), seed 0" % n,
-        "pixels": px, "tile": tile, "threads": threads,
+        "pixels": px, "tile": tile, "threads": threads, "per_image_tile": ptile, "per_image_threads": pthreads,
         "batched_gpx": round(px / (b_ms * 1e-3) / 1e9, 3), "batched_ms": round(b_ms, 3),
         "per_image_launch_gpx": round(px / (p_ms * 1e-3) / 1e9, 3), "per_image_launch_ms": round(p_ms, 3),
         "speedup": round(p_ms / b_ms, 3),

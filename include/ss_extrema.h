@@ -119,8 +119,9 @@ typedef struct ss_image_desc {
 ss_status ss_batch_workspace_size(const ss_image_desc* images, int n, const ss_config* cfg,
                                   size_t* bytes);
 
-/* Runs the whole batch: one persistent tile-stage launch fed by a device-side
- * tile work queue, then the seam stages over all images at once.  `images` is
+/* Runs the whole batch: one tile-stage launch over the tiles of all images
+ * (one workgroup per tile of the combined tile space), then the seam stages
+ * over all images at once.  `images` is
  * a HOST array; its descriptors are staged into the workspace with one
  * asynchronous copy.  Not capturable (the copy reads pageable memory); use
  * ss_batch_plan_* for graph replay. */
@@ -148,6 +149,13 @@ ss_status ss_batch_plan_destroy(ss_batch_plan* plan);
  * the winner's pixels/s. */
 ss_status ss_autotune(const ss_image_desc* images, int n, int reps, hipStream_t stream,
                       ss_config* best, double* throughput);
+
+/* Same, scoring each candidate by the whole batch: total pixels / median time
+ * of ss_extrema_batch_u8 over all n images (the launch a batch or a plan
+ * will use; per-image launch overhead does not enter).  No counterpart in the
+ * reference, whose study times single images only. */
+ss_status ss_autotune_batch(const ss_image_desc* images, int n, int reps, hipStream_t stream,
+                            ss_config* best, double* throughput);
 
 /* Sets / reads the process-wide default config (used when cfg == NULL). */
 ss_status ss_set_default_config(const ss_config* cfg);

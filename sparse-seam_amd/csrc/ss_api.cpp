@@ -115,7 +115,7 @@ ss_status ss_extrema_u8(const uint8_t* d_img, size_t pitch, int width, int heigh
     ims.table = nullptr;
     ims.count = 1;
     const Ws ws = make_ws(d_ws, L, c.threads);
-    const hipError_t e = launch_pipeline(ims, ws, c.tile, c.threads, false, device_cus(), stream);
+    const hipError_t e = launch_pipeline(ims, ws, c.tile, c.threads, device_cus(), stream);
     return e == hipSuccess ? SS_OK : SS_ERR_HIP;
 }
 
@@ -201,7 +201,7 @@ static ss_status batch_enqueue(const ss_image_desc* images, int n, void* d_ws, s
         ims.table = dst;
     }
     const Ws ws = make_ws(d_ws, L, c.threads);
-    const hipError_t e = launch_pipeline(ims, ws, c.tile, c.threads, true, device_cus(), stream);
+    const hipError_t e = launch_pipeline(ims, ws, c.tile, c.threads, device_cus(), stream);
     return e == hipSuccess ? SS_OK : SS_ERR_HIP;
 }
 

@@ -78,9 +78,7 @@ struct WsHeader {
     uint32_t nblocks;      // scan blocks
     uint64_t retries;      // failed CAS in the seam union-find
     uint64_t tiles;
-    uint32_t queue[NUM_XCD];   // per-XCD work-queue tickets (persistent tile stage)
-    uint32_t queue_end[NUM_XCD];
-    uint32_t pad[64 - 10 - 2 * NUM_XCD - 2];
+    uint32_t pad[64 - 10 - 2];
 };
 static_assert(sizeof(WsHeader) == 256, "header must be 256 bytes");
 

@@ -128,7 +128,6 @@ struct ProfileHook {
 ProfileHook& profile_hook();
 
 // Enqueues the whole pipeline on `s`.
-hipError_t launch_pipeline(const Images& ims, const Ws& ws, int T, int NT, bool persistent,
-                           int cus, hipStream_t s);
+hipError_t launch_pipeline(const Images& ims, const Ws& ws, int T, int NT, int cus, hipStream_t s);
 
 }  // namespace ssx

@@ -27,36 +27,6 @@ __global__ __launch_bounds__(NT) void k_tile_stage(Images ims, Ws ws) {
     tile_stage<T, NT>(sm, im, gtile, ws);
 }
 
-// Persistent form: a fixed grid drains a device-side tile work queue.  The
-// tile range is split into one contiguous queue per XCD; a workgroup drains
-// its own XCD's queue first and then steals from the others, so neighbouring
-// tiles share an L2 while the load still balances across the chip.
-template <int T, int NT, bool MULTI>
-__global__ __launch_bounds__(NT) void k_tile_stage_persistent(Images ims, Ws ws, int chunk) {
-    __shared__ TileSmem<T> sm;
-    const int home = blockIdx.x % NUM_XCD;
-#pragma unroll 1
-    for (int q = 0; q < NUM_XCD; ++q) {
-        const int xq = (home + q) % NUM_XCD;
-        for (;;) {
-            __syncthreads();
-            if (threadIdx.x == 0) sm.next = atomicAdd(&ws.hdr->queue[xq], (uint32_t)chunk);
-            __syncthreads();
-            // values read from LDS are not known to be uniform; make them scalar
-            const uint32_t start = __builtin_amdgcn_readfirstlane(sm.next);
-            const uint32_t end_q = ws.hdr->queue_end[xq];
-            if (start >= end_q) break;
-            const uint32_t stop = min(start + (uint32_t)chunk, end_q);
-#pragma unroll 1
-            for (uint32_t t = start; t < stop; ++t) {
-                const ImgDev im = locate<MULTI>(ims, ws.tile_img, t);
-                tile_stage<T, NT>(sm, im, t, ws);
-                __syncthreads();
-            }
-        }
-    }
-}
-
 // Per-tile and per-merge-block image index of a batch: one workgroup per image.
 __global__ __launch_bounds__(256) void k_tile_map(Images ims, Ws ws) {
     const ImgDev im = ims.table[blockIdx.x];
@@ -64,18 +34,6 @@ __global__ __launch_bounds__(256) void k_tile_map(Images ims, Ws ws) {
     for (int64_t t = threadIdx.x; t < n; t += blockDim.x) ws.tile_img[im.tile_begin + t] = blockIdx.x;
     const int64_t nb = (int64_t)im.blocks_x * im.blocks_y;
     for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) ws.block_img[im.block_begin + b] = blockIdx.x;
-}
-
-// Resets the work queues of the persistent tile stage (graph-safe: a kernel,
-// not a memset node).
-__global__ void k_queue_reset(Ws ws) {
-    const int i = threadIdx.x;
-    if (i < NUM_XCD) {
-        const int64_t n = ws.ntiles;
-        const int64_t b = n * i / NUM_XCD, e = n * (i + 1) / NUM_XCD;
-        ws.hdr->queue[i] = (uint32_t)b;
-        ws.hdr->queue_end[i] = (uint32_t)e;
-    }
 }
 
 // ---- compaction: exclusive scan of the per-tile counts --------------------
@@ -576,18 +534,12 @@ __global__ __launch_bounds__(256) void k_writeback(Images ims, Ws ws) {
 
 // ---- host sequencer -------------------------------------------------------
 
+// One workgroup per tile over the combined tile space of all images: for a
+// batch the hardware dispatcher is the tile work queue (a persistent grid
+// draining per-XCD queues needed 148 VGPRs per lane and ran slower).
 template <int T, int NT, bool MULTI>
-static void launch_tile_kernels(const Images& ims, const Ws& ws, bool persistent, int grid_cap,
-                                hipStream_t s) {
-    if (persistent) {
-        k_queue_reset<<<1, 64, 0, s>>>(ws);
-        const int64_t grid = std::min<int64_t>(ws.ntiles, grid_cap);
-        // claims of several tiles keep the queue atomics off the critical path
-        const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(32, ws.ntiles / (grid * 16)));
-        k_tile_stage_persistent<T, NT, MULTI><<<(unsigned)grid, NT, 0, s>>>(ims, ws, chunk);
-    } else {
-        k_tile_stage<T, NT, MULTI><<<(unsigned)ws.ntiles, NT, 0, s>>>(ims, ws);
-    }
+static void launch_tile_kernels(const Images& ims, const Ws& ws, hipStream_t s) {
+    k_tile_stage<T, NT, MULTI><<<(unsigned)ws.ntiles, NT, 0, s>>>(ims, ws);
 }
 
 template <int T, bool MULTI>
@@ -621,13 +573,12 @@ static void launch_merge(const Images& ims, const Ws& ws, hipStream_t s) {
     }
 
 template <bool MULTI>
-static hipError_t launch_all(const Images& ims, const Ws& ws, int T, int NT, bool persistent,
-                             int cus, hipStream_t s) {
+static hipError_t launch_all(const Images& ims, const Ws& ws, int T, int NT, int cus, hipStream_t s) {
     const int grid_cap = std::max(1, cus) * 8;
     if (MULTI) k_tile_map<<<(unsigned)ims.count, 256, 0, s>>>(ims, ws);
     const ProfileHook& hook = profile_hook();
     if (hook.tile_begin) (void)hipEventRecord(hook.tile_begin, s);
-#define CALL_TILE(a, b) launch_tile_kernels<a, b, MULTI>(ims, ws, persistent, grid_cap, s)
+#define CALL_TILE(a, b) launch_tile_kernels<a, b, MULTI>(ims, ws, s)
     SSX_DISPATCH(T, NT, CALL_TILE)
 #undef CALL_TILE
     if (hook.tile_end) (void)hipEventRecord(hook.tile_end, s);
@@ -654,10 +605,9 @@ static hipError_t launch_all(const Images& ims, const Ws& ws, int T, int NT, boo
     return hipGetLastError();
 }
 
-hipError_t launch_pipeline(const Images& ims, const Ws& ws, int T, int NT, bool persistent,
-                           int cus, hipStream_t s) {
-    if (ims.count > 1) return launch_all<true>(ims, ws, T, NT, persistent, cus, s);
-    return launch_all<false>(ims, ws, T, NT, persistent, cus, s);
+hipError_t launch_pipeline(const Images& ims, const Ws& ws, int T, int NT, int cus, hipStream_t s) {
+    if (ims.count > 1) return launch_all<true>(ims, ws, T, NT, cus, s);
+    return launch_all<false>(ims, ws, T, NT, cus, s);
 }
 
 }  // namespace ssx

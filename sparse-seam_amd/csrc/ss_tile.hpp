@@ -116,7 +116,6 @@ struct TileSmem {
     uint8_t eqb[T];                   // cross-seam equality bits, bottom row
     uint32_t count;
     uint32_t err;
-    uint32_t next;                    // work-queue broadcast
     uint32_t zflags;                  // uniform tile: OR of the evidence
     uint32_t open;                    // the tile holds a fragment that may still change
 };

@@ -64,12 +64,13 @@ def _lib():
     lib.ss_batch_plan_run.argtypes = [vp, vp]
     lib.ss_batch_plan_destroy.argtypes = [vp]
     lib.ss_autotune.argtypes = [descp, i, i, vp, cfgp, ctypes.POINTER(ctypes.c_double)]
+    lib.ss_autotune_batch.argtypes = [descp, i, i, vp, cfgp, ctypes.POINTER(ctypes.c_double)]
     lib.ss_profile_events.argtypes = [vp, vp]
     lib.ss_set_default_config.argtypes = [cfgp]
     lib.ss_get_default_config.argtypes = [cfgp]
     for name in ("ss_workspace_size", "ss_extrema_u8", "ss_read_stats", "ss_read_tile_counts",
                  "ss_batch_workspace_size", "ss_extrema_batch_u8", "ss_batch_plan_create",
-                 "ss_batch_plan_run", "ss_batch_plan_destroy", "ss_autotune",
+                 "ss_batch_plan_run", "ss_batch_plan_destroy", "ss_autotune", "ss_autotune_batch",
                  "ss_set_default_config", "ss_get_default_config", "ss_profile_events"):
         getattr(lib, name).restype = i
     _LIB = lib
@@ -330,9 +331,14 @@ def profile_events(begin=None, end=None):
     _check(_lib().ss_profile_events(b, e), "ss_profile_events")
 
 
-def autotune(desc_array, reps=5, stream=None):
+def autotune(desc_array, reps=5, stream=None, batch=False):
+    """Times every (tile, threads) candidate and installs the best as the
+    process default.  batch=False: per-image launches scored by
+    sum(px) / sum(per-image median) (the reference's rule); batch=True: one
+    batched launch over all the images, scored by sum(px) / median."""
     best = Config()
     rate = ctypes.c_double()
-    _check(_lib().ss_autotune(desc_array, len(desc_array), reps, _stream_handle(stream),
-                              ctypes.byref(best), ctypes.byref(rate)), "ss_autotune")
+    fn = "ss_autotune_batch" if batch else "ss_autotune"
+    _check(getattr(_lib(), fn)(desc_array, len(desc_array), reps, _stream_handle(stream),
+                               ctypes.byref(best), ctypes.byref(rate)), fn)
     return (best.tile, best.threads), rate.value
