@@ -6,7 +6,7 @@
 // A scheduler thread admits pending requests in arrival order while their
 // blocks fit, folds them into one ss_extrema_batch_u8 launch on a free stream
 // (H2D copies, pipeline, D2H copies, event), and a completion thread retires
-// launches, frees their blocks and wakes waiters.  With two streams the copies
+// launches in the order they finish, frees their blocks and wakes waiters.  With two streams the copies
 // of one launch overlap the kernels of the other, and requests arriving while
 // launches run are folded into the next one.
 #include <hip/hip_runtime.h>
@@ -216,7 +216,25 @@ void ss_server::reap_loop() {
             cv_done.wait(lk);
             continue;
         }
-        Launch L = inflight.front();
+        // Retire whichever launch has finished first: launches on different
+        // streams complete out of order, and a small batch must not wait
+        // behind a large one's copies.
+        size_t pick = inflight.size();
+        for (size_t i = 0; i < inflight.size(); ++i) {
+            const Launch& c = inflight[i];
+            if (!c.done || hipEventQuery(c.done) != hipErrorNotReady) {
+                pick = i;
+                break;
+            }
+        }
+        if (pick == inflight.size()) {
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            lk.lock();
+            continue;
+        }
+        Launch L = inflight[pick];
+        inflight.erase(inflight.begin() + (std::ptrdiff_t)pick);
         lk.unlock();
         ss_status status = L.status;
         if (L.done && hipEventSynchronize(L.done) != hipSuccess) status = SS_ERR_HIP;
@@ -228,7 +246,6 @@ void ss_server::reap_loop() {
         if (L.done) (void)hipEventDestroy(L.done);
         const double t = now_s();
         lk.lock();
-        inflight.pop_front();
         arena.release(L.ws_off, L.ws_bytes);
         stream_busy[L.stream] = 0;
         for (uint64_t id : L.ids) {
