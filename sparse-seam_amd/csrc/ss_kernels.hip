@@ -180,8 +180,8 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
     __shared__ uint32_t own2[CAP / 16];   // 2-bit local evidence per fragment
     __shared__ uint32_t acc2[CAP / 16];   // 2-bit OR over the component, at its root
     const ImgDev im = MULTI ? ims.table[ws.block_img[blockIdx.x]] : ims.one;
-    const int64_t bl = (int64_t)blockIdx.x - im.block_begin;
-    const int tx0 = (int)(bl % im.blocks_x) * MERGE_BT, ty0 = (int)(bl / im.blocks_x) * MERGE_BT;
+    const uint32_t bl = (uint32_t)((int64_t)blockIdx.x - im.block_begin);
+    const int tx0 = (int)(bl % (uint32_t)im.blocks_x) * MERGE_BT, ty0 = (int)(bl / (uint32_t)im.blocks_x) * MERGE_BT;
     const int ntx = min(MERGE_BT, im.tiles_x - tx0), nty = min(MERGE_BT, im.tiles_y - ty0);
     auto gtile = [&](int i) -> int64_t {
         return im.tile_begin + (int64_t)(ty0 + i / MERGE_BT) * im.tiles_x + tx0 + i % MERGE_BT;
@@ -386,8 +386,8 @@ __global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
     uint32_t ba = NONE, bb[3] = {NONE, NONE, NONE};   // bottom edge
     if (gtile < ws.ntiles) {
         const ImgDev im = locate<MULTI>(ims, ws.tile_img, gtile);
-        const int64_t lt = gtile - im.tile_begin;
-        const int tx = (int)(lt % im.tiles_x), ty = (int)(lt / im.tiles_x);
+        const uint32_t lt = (uint32_t)(gtile - im.tile_begin);
+        const int tx = (int)(lt % (uint32_t)im.tiles_x), ty = (int)(lt / (uint32_t)im.tiles_x);
         // pairs inside one merge block were united by k_block_merge
         const int bxm = tx % MERGE_BT, bym = ty % MERGE_BT;
         const bool right_cross = bxm == MERGE_BT - 1 || (k == 0 && bym == 0) || (k == T - 1 && bym == MERGE_BT - 1);

@@ -402,7 +402,10 @@ __device__ __forceinline__ bool perim_pixel(int e, int tw, int th, int& lx, int&
 
 // Exact seam tests of the thread's perimeter slots: an equal neighbour that is
 // outside the tile but inside the image.  Writes the cross-seam equality bits
-// of the right column and bottom row; returns one seam bit per slot.
+// of the right column and bottom row; returns one seam bit per slot.  The nine
+// bytes around a perimeter pixel all lie in the staged rows -1 .. th and
+// columns -1 .. tw, so they are read unconditionally (one LDS round trip) and
+// the neighbour rules are applied as masks.
 template <int T, int NT>
 __device__ __forceinline__ uint32_t perimeter_pass(TileSmem<T>& sm, const ImgDev& im, int x0, int y0, int tw,
                                                    int th) {
@@ -416,18 +419,19 @@ __device__ __forceinline__ uint32_t perimeter_pass(TileSmem<T>& sm, const ImgDev
         if (e >= 4 * T || !perim_pixel<T>(e, tw, th, lx, ly)) continue;
         const uint8_t* c = &sm.img[(ly + 1) * P + 4 + lx];
         const uint32_t v = c[0];
-        auto sq = [&](int dx, int dy) -> uint32_t {
-            const int nx = lx + dx, ny = ly + dy;
-            const bool out = nx < 0 || nx >= tw || ny < 0 || ny >= th;
-            const bool in_img = (nx >= 0 || hasL) && (nx < tw || hasR) && (ny >= 0 || hasU) && (ny < th || hasD);
-            return (out && in_img && c[dy * P + dx] == v) ? 1u : 0u;
-        };
-        const uint32_t sul = sq(-1, -1), su = sq(0, -1), sur = sq(1, -1);
-        const uint32_t sl = sq(-1, 0), sr = sq(1, 0);
-        const uint32_t sdl = sq(-1, 1), sd = sq(0, 1), sdr = sq(1, 1);
+        const uint32_t eul = c[-P - 1] == v, eu = c[-P] == v, eur = c[-P + 1] == v;
+        const uint32_t el = c[-1] == v, er = c[1] == v;
+        const uint32_t edl = c[P - 1] == v, ed = c[P] == v, edr = c[P + 1] == v;
+        // a neighbour counts if it is outside the tile in some direction and
+        // inside the image in both
+        const bool oL = lx == 0, oR = lx == tw - 1, oU = ly == 0, oD = ly == th - 1;
+        const bool iL = !oL || hasL, iR = !oR || hasR, iU = !oU || hasU, iD = !oD || hasD;
+        const uint32_t sul = eul & ((oL || oU) && iL && iU), su = eu & (oU && iU), sur = eur & ((oR || oU) && iR && iU);
+        const uint32_t sl = el & (oL && iL), sr = er & (oR && iR);
+        const uint32_t sdl = edl & ((oL || oD) && iL && iD), sd = ed & (oD && iD), sdr = edr & ((oR || oD) && iR && iD);
         if (sul | su | sur | sl | sr | sdl | sd | sdr) seam_bits |= 1u << k;
-        if (lx == tw - 1) sm.eqr[ly] = (uint8_t)(sur | (sr << 1) | (sdr << 2));
-        if (ly == th - 1) sm.eqb[lx] = (uint8_t)(sdl | (sd << 1) | (sdr << 2));
+        if (oR) sm.eqr[ly] = (uint8_t)(sur | (sr << 1) | (sdr << 2));
+        if (oD) sm.eqb[lx] = (uint8_t)(sdl | (sd << 1) | (sdr << 2));
     }
     return seam_bits;
 }
@@ -564,8 +568,9 @@ __device__ __forceinline__ void write_perimeter_records(TileSmem<T>& sm, uint8_t
 template <int T, int NT>
 __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, int64_t gtile, const Ws& ws) {
     constexpr int S = 4 * T - 4, PER = Quads<T, NT>::PER;
-    const int64_t lt = gtile - im.tile_begin;
-    const int tx = (int)(lt % im.tiles_x), ty = (int)(lt / im.tiles_x);
+    // fragment slots are 32-bit indices, so an image has fewer than 2^32 tiles
+    const uint32_t lt = (uint32_t)(gtile - im.tile_begin);
+    const int tx = (int)(lt % (uint32_t)im.tiles_x), ty = (int)(lt / (uint32_t)im.tiles_x);
     const int x0 = tx * T, y0 = ty * T;
     const int tw = min(T, im.w - x0), th = min(T, im.h - y0);
     if (threadIdx.x == 0) {
@@ -719,8 +724,8 @@ __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, i
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const int64_t lt = gtile - im.tile_begin;
-    const int tx = (int)(lt % im.tiles_x), ty = (int)(lt / im.tiles_x);
+    const uint32_t lt = (uint32_t)(gtile - im.tile_begin);
+    const int tx = (int)(lt % (uint32_t)im.tiles_x), ty = (int)(lt / (uint32_t)im.tiles_x);
     const int x0 = tx * T, y0 = ty * T;
     const int tw = min(T, im.w - x0), th = min(T, im.h - y0);
     const bool al_out = (((uintptr_t)im.mx | (uintptr_t)im.mn | (uintptr_t)im.opitch) & 3u) == 0;
