@@ -41,7 +41,7 @@ def percentile(values, q):
     return v[lo] + (v[hi] - v[lo]) * (pos - lo)
 
 
-def cpu_baseline(images, threads):
+def cpu_baseline(images, threads, runs=2):
     """Global-UF oracle route on the host cores over a bounded sample."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from sscommon import oracle_global_uf
@@ -51,7 +51,7 @@ def cpu_baseline(images, threads):
         h = img.cpu().numpy()
         oracle_global_uf(h, threads)  # warm-up (page faults, thread pool)
         ts = []
-        for _ in range(2):
+        for _ in range(runs):
             t0 = time.perf_counter()
             oracle_global_uf(h, threads)
             ts.append(time.perf_counter() - t0)
@@ -61,11 +61,22 @@ def cpu_baseline(images, threads):
     return px / med / 1e9, names
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def batched_leg(n, dev, ss, ssgen, torch, reps=3):
     """Mixed-size batch (widths and heights U[256, 1024], content 1:1:1 iid /
     smoothed sigma 3 at 16 levels / EDT of disks): autotuned (tile, threads),
     per-image launches of the plain path versus one hipGraph-replayed batched
-    pipeline (persistent tile stage over a device work queue)."""
+    pipeline (one tile-stage launch over the tiles of all images)."""
     import numpy as np
     t0 = time.time()
     sizes = ssgen.mixed_sizes(n, seed=0)
@@ -371,10 +382,13 @@ def main():
         cores = args.cpu_threads or min(16, os.cpu_count() or 1)
         t0 = time.time()
         rate, names = cpu_baseline(cpu_sample, cores)
+        rate1, _ = cpu_baseline(cpu_sample, 1, runs=1)
         out["cpu_baseline"] = {
             "value": round(rate, 4), "unit": "Gpixel/s", "cores": cores, "kind": "port",
             "sample": "Global-UF oracle route on %s at %dx%d (median of 2 runs each after a "
                       "warm-up, sum(px)/sum(median))" % ("+".join(names), side, side),
+            "single_thread_value": round(rate1, 4),
+            "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
         }
         log("cpu baseline %.1fs" % (time.time() - t0))
     print(json.dumps(out), flush=True)

@@ -29,15 +29,16 @@ constexpr int Q_EL = 8;       // equal left neighbour in the tile
 constexpr int Q_EU = 12;      // equal up neighbour in the tile
 constexpr int Q_EUL = 16;     // equal up-left neighbour in the tile
 constexpr int Q_EUR = 20;     // equal up-right neighbour in the tile
-constexpr int Q_SGL = 24;     // singleton (no equal 8-neighbour)
-constexpr int Q_VAL = 28;     // pixel lies inside the image part of the tile
+constexpr int Q_HD = 24;      // run head (see below)
+constexpr int Q_UN = 28;      // run head with contacts its Komura link does not cover
 
 // Runs: the non-singleton pixels of a quad split into maximal runs of equal
 // horizontal neighbours.  Only a run's first pixel (its head) is a union-find
 // node; the other pixels of the run point at the head and never change.  A
 // plateau quad is one node instead of four.
-__device__ __forceinline__ uint32_t q_nsg(uint32_t st) { return (st >> Q_VAL) & ~(st >> Q_SGL) & 0xFu; }
-__device__ __forceinline__ uint32_t q_heads(uint32_t st) { return q_nsg(st) & ~((st >> Q_EL) & 0xEu); }
+__device__ __forceinline__ uint32_t q_heads(uint32_t st) { return (st >> Q_HD) & 0xFu; }
+// Heads the union pass must visit.
+__device__ __forceinline__ uint32_t q_unions(uint32_t st) { return (st >> Q_UN) & 0xFu; }
 // Pixels of the run that starts at head h.
 __device__ __forceinline__ uint32_t q_run(uint32_t st, int h) {
     const uint32_t c = (st >> Q_EL) & 0xEu;
@@ -153,6 +154,10 @@ constexpr uint32_t B_G = 0x80808080u;     // a strictly greater 8-neighbour exis
 constexpr uint32_t B_S = 0x40404040u;     // a strictly smaller 8-neighbour exists
 constexpr uint32_t B_SGL = 0x20202020u;   // singleton
 constexpr uint32_t B_VAL = 0x10101010u;   // pixel inside the image part of the tile
+constexpr uint32_t B_RT = 0x01010101u;    // run head whose Komura link is itself
+// The four B_RT bits of a byte-form word as a 4-bit mask: the only heads that
+// can end as roots (a union only ever turns a root into a non-root).
+__device__ __forceinline__ uint32_t b_roots(uint32_t sb) { return (((sb & B_RT) * 0x01020408u) >> 24) & 0xFu; }
 // 0x80-per-byte flags -> 0xFF bytes.
 __device__ __forceinline__ uint32_t bytes80(uint32_t x) { return (x - (x >> 7)) | x; }
 // Two 0x80-per-byte flag words -> one byte: a's pixels in bits 0-3, b's in 4-7.
@@ -318,15 +323,17 @@ __device__ __forceinline__ uint32_t classify_quad(TileSmem<T>& sm, const RowW& a
     const uint32_t sgl80 = neq & val80;
     all_eq = all_eq && (ne_top(V, v4) & val80) == 0;
     st = 0;
+    uint32_t rtb = 0;
     if (val80 & ~neq) {
         const uint32_t lv80 = lx0 > 0 ? 0x80808080u : 0x80808000u;   // left neighbour in the tile
         const uint32_t rv80 = first_bytes80(tw - lx0 - 1);              // right neighbour in the tile
         const uint32_t up80 = ly > 0 ? val80 : 0u;
         const uint32_t eu80 = ~nU & up80, eul80 = ~nUL & up80 & lv80, eur80 = ~nUR & up80 & rv80;
         const uint32_t el80 = ~nL & lv80 & val80;
-        const uint32_t s = movemask_pair(g80, s80) << Q_G | movemask_pair(el80, eu80) << Q_EL |
-                           movemask_pair(eul80, eur80) << Q_EUL | movemask_pair(sgl80, val80) << Q_SGL;
-        st = s;
+        uint32_t s = movemask_pair(g80, s80) << Q_G | movemask_pair(el80, eu80) << Q_EL |
+                     movemask_pair(eul80, eur80) << Q_EUL;
+        const uint32_t nsg4 = movemask_pair(val80 & ~neq, 0u);
+        s |= (nsg4 & ~((s >> Q_EL) & 0xEu)) << Q_HD;
         // Labels of the quad in one 16-byte store: a head links to its first
         // contact above (else to the pixel left of the quad, else to itself),
         // every other run pixel to its head.  Singletons get a dummy entry
@@ -336,19 +343,28 @@ __device__ __forceinline__ uint32_t classify_quad(TileSmem<T>& sm, const RowW& a
         const uint32_t el4 = (s >> Q_EL) & 0xFu;
         uint32_t lab4[4];
         uint32_t head = p0;
+        uint32_t un = 0, rt = 0;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             if ((hd >> i) & 1u) {
                 head = p0 + i;
                 const uint32_t ab = q_above(s, q_run(s, i));
-                lab4[i] = ab ? (p0 - T + __builtin_ctz(ab) - 1u) : ((i == 0 && (el4 & 1u)) ? p0 - 1u : p0 + i);
+                const bool left = i == 0 && (el4 & 1u);
+                lab4[i] = ab ? (p0 - T + __builtin_ctz(ab) - 1u) : (left ? p0 - 1u : p0 + i);
+                // contact segments beyond the first, or a left neighbour the
+                // link does not reach (see quad_label)
+                const uint32_t seg = ab & ~(ab << 1);
+                if ((seg & (seg - 1u)) || (left && ab && !(ab & 3u))) un |= 1u << i;
+                if (!ab && !left) rt |= 1u << i;
             } else {
                 lab4[i] = head;   // run pixel (or an unread dummy)
             }
         }
+        st = s | un << Q_UN;
+        rtb = (rt * 0x00204081u) & B_RT;
         *reinterpret_cast<uint4*>(&sm.lab[p0]) = make_uint4(lab4[0], lab4[1], lab4[2], lab4[3]);
     }
-    return g80 | (s80 >> 1) | (sgl80 >> 2) | (val80 >> 3);
+    return g80 | (s80 >> 1) | (sgl80 >> 2) | (val80 >> 3) | rtb;
 }
 
 // First pass over the thread's quads (a vertical strip): evidence, singleton
@@ -444,14 +460,15 @@ __device__ __forceinline__ uint32_t perimeter_pass(TileSmem<T>& sm, const ImgDev
 // evidence of the perimeter slots, into the roots.  Assumes a barrier after
 // quad_init; ends with one.
 template <int T, int NT>
-__device__ __forceinline__ void quad_label(TileSmem<T>& sm, const uint32_t (&st)[Quads<T, NT>::PER], int tw,
-                                           int th, uint32_t seam_bits) {
+__device__ __forceinline__ void quad_label(TileSmem<T>& sm, const uint32_t (&st)[Quads<T, NT>::PER],
+                                           const uint32_t (&sb)[Quads<T, NT>::PER], int tw, int th,
+                                           uint32_t seam_bits) {
 #pragma unroll
     for (int k = 0; k < Quads<T, NT>::PER; ++k) {
         int ly, lx0;
         Quads<T, NT>::at(k, ly, lx0);
         const int p0 = ly * T + lx0;
-        uint32_t hd = q_heads(st[k]);
+        uint32_t hd = q_heads(st[k]) & ~b_roots(sb[k]);   // a self-linked head is already compressed
         while (hd) {
             const int i = __builtin_ctz(hd);
             hd &= hd - 1u;
@@ -464,13 +481,12 @@ __device__ __forceinline__ void quad_label(TileSmem<T>& sm, const uint32_t (&st)
         int ly, lx0;
         Quads<T, NT>::at(k, ly, lx0);
         const int p0 = ly * T + lx0;
-        uint32_t hd = q_heads(st[k]);
+        uint32_t hd = q_unions(st[k]);
         while (hd) {
             const int i = __builtin_ctz(hd);
             hd &= hd - 1u;
             const uint32_t ab = q_above(st[k], q_run(st[k], i));
             uint32_t seg = ab & ~(ab << 1);
-            if (!seg) continue;   // linked left or alone: nothing more to unite
             seg &= seg - 1u;      // the first segment is the head's link
             while (seg) {
                 const int j = __builtin_ctz(seg);
@@ -617,14 +633,14 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
         }
         write_perimeter_records<T, NT>(sm, rec, tw, th, seam_bits, true);
     } else {
-        quad_label<T, NT>(sm, st, tw, th, seam_bits);
+        quad_label<T, NT>(sm, st, sb, tw, th, seam_bits);
         // Number the seam zones (their roots) and fill their fragment slots.
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             int ly, lx0;
             Quads<T, NT>::at(k, ly, lx0);
             const int p0 = ly * T + lx0;
-            uint32_t hd = q_heads(st[k]);
+            uint32_t hd = b_roots(sb[k]);
             while (hd) {
                 const int i = __builtin_ctz(hd);
                 hd &= hd - 1u;

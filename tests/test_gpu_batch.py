@@ -1,4 +1,4 @@
-"""Batched form (one persistent tile-stage launch over many uneven images),
+"""Batched form (one tile-stage launch over the tiles of many uneven images),
 its hipGraph-captured plan, and the autotuner -- all through the C ABI, all
 checked bit for bit against the single-image path and the oracle."""
 import sys
