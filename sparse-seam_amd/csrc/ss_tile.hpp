@@ -148,6 +148,60 @@ struct Quads {
     }
 };
 
+// The calling thread's quads as one bitmask (bit 4k + i = pixel i of quad k),
+// so that a pass over the heads of all its quads runs as many iterations as
+// the thread has heads, not as the sum over k of the wave's busiest lane.
+template <int PER>
+struct QuadMask {
+    using type = uint32_t;
+};
+template <>
+struct QuadMask<16> {
+    using type = uint64_t;
+};
+template <int T, int NT>
+struct QuadSet {
+    static constexpr int PER = Quads<T, NT>::PER;
+    using M = typename QuadMask<PER>::type;
+    // Raster index of pixel 0 of quad 0; quad k starts T * k further down.
+    __device__ static __forceinline__ uint32_t base() {
+        int ly, lx0;
+        Quads<T, NT>::at(0, ly, lx0);
+        return (uint32_t)(ly * T + lx0);
+    }
+    template <typename G>
+    __device__ static __forceinline__ M gather(G&& nibble) {
+        M m = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) m |= (M)nibble(k) << (4 * k);
+        return m;
+    }
+    // Calls f(k, i) for every set bit.
+    template <typename F>
+    __device__ static __forceinline__ void each(M m, F&& f) {
+        while (m) {
+            const int b = sizeof(M) == 8 ? __builtin_ctzll((uint64_t)m) : __builtin_ctz((uint32_t)m);
+            m &= m - 1;
+            f(b >> 2, b & 3);
+        }
+    }
+    // v[k] for a k known only at run time, as a chain of selects between
+    // registers.  The empty asm makes each operand an opaque register value:
+    // otherwise the compiler turns the chain into an indexed load and keeps
+    // the whole array in private memory.
+    __device__ static __forceinline__ uint32_t pick(const uint32_t (&v)[PER], int k) {
+        uint32_t r = v[0];
+        __asm__("" : "+v"(r));
+#pragma unroll
+        for (int j = 1; j < PER; ++j) {
+            uint32_t x = v[j];
+            __asm__("" : "+v"(x));
+            r = k == j ? x : r;
+        }
+        return r;
+    }
+};
+
 // Per-quad evidence in byte form (bit b of byte i = flag of pixel i), the
 // form the classification produces and the mask stores consume.
 constexpr uint32_t B_G = 0x80808080u;     // a strictly greater 8-neighbour exists
@@ -463,63 +517,41 @@ template <int T, int NT>
 __device__ __forceinline__ void quad_label(TileSmem<T>& sm, const uint32_t (&st)[Quads<T, NT>::PER],
                                            const uint32_t (&sb)[Quads<T, NT>::PER], int tw, int th,
                                            uint32_t seam_bits) {
-#pragma unroll
-    for (int k = 0; k < Quads<T, NT>::PER; ++k) {
-        int ly, lx0;
-        Quads<T, NT>::at(k, ly, lx0);
-        const int p0 = ly * T + lx0;
-        uint32_t hd = q_heads(st[k]) & ~b_roots(sb[k]);   // a self-linked head is already compressed
-        while (hd) {
-            const int i = __builtin_ctz(hd);
-            hd &= hd - 1u;
-            sm.lab[p0 + i] = lfind_h(sm.lab, p0 + i);
-        }
-    }
+    using QS = QuadSet<T, NT>;
+    const uint32_t base = QS::base();
+    // a self-linked head is already compressed
+    QS::each(QS::gather([&](int k) { return q_heads(st[k]) & ~b_roots(sb[k]); }), [&](int k, int i) {
+        const uint32_t p = base + k * T + i;
+        sm.lab[p] = lfind_h(sm.lab, p);
+    });
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < Quads<T, NT>::PER; ++k) {
-        int ly, lx0;
-        Quads<T, NT>::at(k, ly, lx0);
-        const int p0 = ly * T + lx0;
-        uint32_t hd = q_unions(st[k]);
-        while (hd) {
-            const int i = __builtin_ctz(hd);
-            hd &= hd - 1u;
-            const uint32_t ab = q_above(st[k], q_run(st[k], i));
-            uint32_t seg = ab & ~(ab << 1);
-            seg &= seg - 1u;      // the first segment is the head's link
-            while (seg) {
-                const int j = __builtin_ctz(seg);
-                seg &= seg - 1u;
-                lunion_h(sm.lab, p0 + i, p0 - T + j - 1);
-            }
-            // The left pixel needs a union only if the run touches neither of
-            // its two upper neighbours: such a contact is one of the left
-            // pixel's own contacts, to which both runs are already joined.
-            if (i == 0 && ((st[k] >> Q_EL) & 1u) && !(ab & 3u)) lunion_h(sm.lab, p0, p0 - 1);
+    QS::each(QS::gather([&](int k) { return q_unions(st[k]); }), [&](int k, int i) {
+        const uint32_t s = QS::pick(st, k), p0 = base + k * T;
+        const uint32_t ab = q_above(s, q_run(s, i));
+        uint32_t seg = ab & ~(ab << 1);
+        seg &= seg - 1u;      // the first segment is the head's link
+        while (seg) {
+            const int j = __builtin_ctz(seg);
+            seg &= seg - 1u;
+            lunion_h(sm.lab, p0 + i, p0 - T + j - 1);
         }
-    }
+        // The left pixel needs a union only if the run touches neither of
+        // its two upper neighbours: such a contact is one of the left
+        // pixel's own contacts, to which both runs are already joined.
+        if (i == 0 && ((s >> Q_EL) & 1u) && !(ab & 3u)) lunion_h(sm.lab, p0, p0 - 1);
+    });
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < Quads<T, NT>::PER; ++k) {
-        int ly, lx0;
-        Quads<T, NT>::at(k, ly, lx0);
-        const int p0 = ly * T + lx0;
-        uint32_t hd = q_heads(st[k]);
-        while (hd) {
-            const int i = __builtin_ctz(hd);
-            hd &= hd - 1u;
-            const uint32_t p = p0 + i;
-            const uint32_t r = lfindm(sm.lab, p);
-            if (r != p) sm.lab[p] = r;
-            const uint32_t run = q_run(st[k], i);
-            const uint32_t f = (((st[k] >> Q_G) & run) ? F_G : 0u) | (((st[k] >> Q_S) & run) ? F_S : 0u);
-            // skip the atomic when the root already holds the evidence: every
-            // run of a plateau's rim would otherwise serialise on one word
-            if (f & ~(lds_load(&sm.lab[r]) >> FLAG_SHIFT))
-                __hip_atomic_fetch_or(&sm.lab[r], f << FLAG_SHIFT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-    }
+    QS::each(QS::gather([&](int k) { return q_heads(st[k]); }), [&](int k, int i) {
+        const uint32_t s = QS::pick(st, k), p = base + k * T + i;
+        const uint32_t r = lfindm(sm.lab, p);
+        if (r != p) sm.lab[p] = r;
+        const uint32_t run = q_run(s, i);
+        const uint32_t f = (((s >> Q_G) & run) ? F_G : 0u) | (((s >> Q_S) & run) ? F_S : 0u);
+        // skip the atomic when the root already holds the evidence: every
+        // run of a plateau's rim would otherwise serialise on one word
+        if (f & ~(lds_load(&sm.lab[r]) >> FLAG_SHIFT))
+            __hip_atomic_fetch_or(&sm.lab[r], f << FLAG_SHIFT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    });
 #pragma unroll
     for (int k = 0; k < Quads<T, NT>::PPER; ++k) {
         if (!((seam_bits >> k) & 1u)) continue;
@@ -635,28 +667,23 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
     } else {
         quad_label<T, NT>(sm, st, sb, tw, th, seam_bits);
         // Number the seam zones (their roots) and fill their fragment slots.
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            int ly, lx0;
-            Quads<T, NT>::at(k, ly, lx0);
-            const int p0 = ly * T + lx0;
-            uint32_t hd = b_roots(sb[k]);
-            while (hd) {
-                const int i = __builtin_ctz(hd);
-                hd &= hd - 1u;
-                const uint32_t p = p0 + i;
+        {
+            using QS = QuadSet<T, NT>;
+            const uint32_t base = QS::base();
+            QS::each(QS::gather([&](int k) { return b_roots(sb[k]); }), [&](int k, int i) {
+                const uint32_t p = base + k * T + i;
                 const uint32_t w = lds_load(&sm.lab[p]);
-                if ((w & LMASK) != p || !((w >> FLAG_SHIFT) & F_SEAM)) continue;
+                if ((w & LMASK) != p || !((w >> FLAG_SHIFT) & F_SEAM)) return;
                 const uint32_t j = atomicAdd(&sm.count, 1u);
                 if (j >= (uint32_t)S) {
                     sm.err = 1u;
-                    continue;
+                    return;
                 }
                 sm.lab[p] = w | ((j + 1) << FRAG_SHIFT);
                 ws.sbits[gtile * S + j] = (uint8_t)((w >> FLAG_SHIFT) & 3u);
                 // a fragment that already has both kinds of evidence is final
                 if (((w >> FLAG_SHIFT) & 3u) != 3u) sm.open = 1u;
-            }
+            });
         }
         __syncthreads();
         // Masks (final for tile-contained zones, provisional for seam zones)
