@@ -285,8 +285,8 @@ def main():
         for _, img in images:
             ss.extrema(img, tile, threads, ws, (mx, mn))
     torch.cuda.synchronize()
-    st = ss.stats(ws)
-    log("warm-up done", st)
+    if args.warmup > 0:
+        log("warm-up done", ss.stats(ws))
 
     n = len(images)
     K = args.steps
@@ -314,6 +314,7 @@ def main():
     torch.cuda.synchronize()
     ss.profile_events(None, None)
     total_ms = start.elapsed_time(stop)
+    st = ss.stats(ws)   # of the last timed image
 
     per_img = [[img_ev[k][i][0].elapsed_time(img_ev[k][i][1]) for k in range(K)] for i in range(n)]
     k1 = [k1_ev[k][i][0].elapsed_time(k1_ev[k][i][1]) for k in range(K) for i in range(n)]

@@ -89,7 +89,10 @@ constexpr uint32_t COUNT_MASK = 0x00FFFFFFu;
 constexpr uint32_t CNT_UNIFORM = 1u << 24;   // the tile is one zone (fragment 0 if seam)
 constexpr uint32_t CNT_OPEN = 1u << 25;      // the tile wrote its per-pixel record plane
 constexpr int COUNT_ERR_SHIFT = 28;
-constexpr uint8_t REC_NONE = 255;            // record of a pixel outside every open fragment
+// Record-plane byte of a pixel outside every open fragment: REC_CLOSED plus
+// its final mask bits (bit 0 max, bit 1 min).  Fragment ids stay below it
+// (at most 4T - 4 = 252 slots, ids 0 .. 251).
+constexpr uint32_t REC_CLOSED = 0xFCu;
 
 struct Ws {
     WsHeader* hdr;
@@ -99,7 +102,7 @@ struct Ws {
     uint32_t* berr;      // [nblocks] OR of the tile invariant codes per scan block
     uint8_t* sbits;      // [tiles * S] local G/S evidence per fragment slot
     uint8_t* rec8;       // [tiles * T*T] per-pixel record (open tiles only): the
-                         //   pixel's open fragment, or REC_NONE
+                         //   pixel's open fragment, or REC_CLOSED | its final masks
     uint8_t* rec;        // [tiles * 4T] fragment id of seam perimeter pixels
                          //   [0,T) top row, [T,2T) bottom row, [2T,3T) left col, [3T,4T) right col
     uint8_t* eq;         // [tiles * 2T] cross-seam equality bits: [0,T) right col, [T,2T) bottom row

@@ -700,7 +700,7 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
             const uint32_t selb = bytes80((b << 3) & B_G);
             // singletons straight from the evidence bits, the others from their root
             uint32_t mxw = bytes80((b << 2) & ~b & B_G), mnw = bytes80((b << 2) & ~(b << 1) & B_G);
-            uint32_t rcw = 0xFFFFFFFFu;
+            uint32_t om = 0, ids = 0;   // bytes of open fragments, and their ids
             uint32_t hd = q_heads(st[k]);
             while (hd) {
                 const int i = __builtin_ctz(hd);
@@ -709,14 +709,20 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
                 const uint32_t rw = sm.lab[sm.lab[ly * T + lx0 + i] & LMASK];
                 const uint32_t zf = rw >> FLAG_SHIFT;
                 const uint32_t jr = (rw >> FRAG_SHIFT) & 0xFFu;
-                if (jr && (zf & 3u) != 3u) rcw ^= (0xFFu ^ (jr - 1u)) * 0x01010101u & rb;
+                if (jr && (zf & 3u) != 3u) {
+                    om |= rb;
+                    ids |= (jr - 1u) * 0x01010101u & rb;
+                }
                 if (!(zf & F_G)) mxw |= rb;
                 if (!(zf & F_S)) mnw |= rb;
             }
             const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
             store_bytes(im.mx + o, mxw, selb, al_out);
             store_bytes(im.mn + o, mnw, selb, al_out);
-            if (open) *reinterpret_cast<uint32_t*>(plane + ly * T + lx0) = rcw;
+            if (open) {
+                const uint32_t closed = REC_CLOSED * 0x01010101u | (mxw & 0x01010101u) | (mnw & 0x02020202u);
+                *reinterpret_cast<uint32_t*>(plane + ly * T + lx0) = (closed & ~om) | ids;
+            }
         }
         write_perimeter_records<T, NT>(sm, rec, tw, th, seam_bits, false);
     }
@@ -736,12 +742,13 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
 // Mark and write-back of one tile by one wave (ss_device.hpp, step 5).
 // cw is the tile's counts word; only a tile with an open fragment, or a
 // uniform tile (whose one fragment may be open), can change.  Lanes resolve
-// the tile's fragments against the seam union-find; a fragment whose resolved
-// evidence adds a bit to its local evidence changed, and its final mask bits
-// go into the wave's LDS table.  If any changed, the wave rewrites only the
-// pixels of changed fragments: a uniform tile is one fragment, any other tile
-// streams its record plane -- no image read, no relabelling.  Returns whether
-// the tile changed.
+// the tile's fragments against the seam union-find into the wave's LDS table
+// (bit 7: the fragment changed; bits 0-1: its final max / min masks).  If any
+// changed, the wave rewrites the tile: a uniform tile is one fragment; any
+// other tile streams its record plane (fragment ids, or the final masks of
+// closed pixels) and rewrites every whole tile row that holds a changed pixel,
+// so HBM sees full row segments rather than scattered partial lines.  No image
+// read, no relabelling.  Returns whether the tile changed.
 template <int T>
 __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, int64_t gtile, uint32_t cw,
                                                const Ws& ws) {
@@ -751,15 +758,11 @@ __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, i
     bool changed = false;
     for (uint32_t j = lane; j < cnt; j += 64) {
         const uint32_t own = ws.dbits[b + j];
-        uint32_t v = 0;
-        if ((own & 3u) != 3u) {
-            const uint32_t res = ((ws.dbits[gfind_ro(ws.dparent, b + j)] >> 2) | own) & 3u;
-            if (res & ~own) {
-                v = 0x80u | ((res & F_G) ? 0u : 1u) | ((res & F_S) ? 0u : 2u);
-                changed = true;
-            }
-        }
-        lut[j] = (uint8_t)v;
+        uint32_t res = own & 3u;
+        if (res != 3u) res |= (ws.dbits[gfind_ro(ws.dparent, b + j)] >> 2) & 3u;
+        const bool ch = (res & ~own & 3u) != 0;
+        changed = changed || ch;
+        lut[j] = (uint8_t)((ch ? 0x80u : 0u) | ((res & F_G) ? 0u : 1u) | ((res & F_S) ? 0u : 2u));
     }
     if (!__ballot(changed)) return false;
     // the table was written by other lanes of this wave: LDS operations of one
@@ -785,44 +788,63 @@ __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, i
         }
         return true;
     }
-    // record plane: 16 bytes per lane and step, 4 quads each
+    // record plane: 16 bytes per lane and step; the VR lanes of one tile row
+    // are consecutive and aligned, so one ballot tells each lane whether its
+    // row changed.  A changed row is rewritten whole: one full segment per
+    // mask instead of partial lines that HBM must read back and merge.  Plane
+    // bytes outside the image part of the tile were never written and count
+    // as closed, unchanged pixels.
     const uint8_t* plane = ws.rec8 + gtile * (int64_t)(T * T);
     constexpr int VR = T / 16;   // 16-byte vectors per row
+    const bool al16 = (((uintptr_t)im.mx | (uintptr_t)im.mn | (uintptr_t)im.opitch) & 15u) == 0;
+    constexpr uint32_t CL4 = REC_CLOSED * 0x01010101u;
     for (int u = lane; u < T * T / 16; u += 64) {
         const int ly = u / VR, c0 = 16 * (u % VR);
-        if (ly >= th || c0 >= tw) continue;
         const uint4 rv = *reinterpret_cast<const uint4*>(plane + ly * T + c0);
-        const uint32_t rw[4] = {rv.x, rv.y, rv.z, rv.w};
-        uint32_t mxw[4], mnw[4], sel[4];
+        uint32_t rw[4] = {rv.x, rv.y, rv.z, rv.w};
+        uint32_t mxw[4], mnw[4];
+        bool ch = false;
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
-            const int lx0 = c0 + 4 * d;
-            mxw[d] = 0;
-            mnw[d] = 0;
-            sel[d] = 0;
+            if (ly >= th) {
+                rw[d] = CL4;
+            } else {
+                const int nv = tw - c0 - 4 * d;   // valid bytes of this dword
+                if (nv < 4) rw[d] = nv <= 0 ? CL4 : (rw[d] & (0xFFFFFFFFu >> (32 - 8 * nv))) | (CL4 << (8 * nv));
+            }
+            const uint32_t w = rw[d];
+            if ((w & CL4) == CL4) {
+                // four closed pixels: final masks straight from the codes
+                mxw[d] = (w & 0x01010101u) * 0xFFu;
+                mnw[d] = ((w >> 1) & 0x01010101u) * 0xFFu;
+            } else {
+                mxw[d] = 0;
+                mnw[d] = 0;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const uint32_t r = (rw[d] >> (8 * i)) & 0xFFu;
-                if (lx0 + i >= tw || r == REC_NONE) continue;
-                const uint32_t v = lut[r];
-                if (!(v & 0x80u)) continue;
-                sel[d] |= 1u << i;
-                mxw[d] |= ((v & 1u) ? 0xFFu : 0u) << (8 * i);
-                mnw[d] |= ((v & 2u) ? 0xFFu : 0u) << (8 * i);
+                for (int i = 0; i < 4; ++i) {
+                    const uint32_t r = (w >> (8 * i)) & 0xFFu;
+                    const uint32_t v = r >= REC_CLOSED ? (r & 3u) : (uint32_t)lut[r];
+                    ch = ch || (v & 0x80u) != 0;
+                    mxw[d] |= ((v & 1u) ? 0xFFu : 0u) << (8 * i);
+                    mnw[d] |= ((v & 2u) ? 0xFFu : 0u) << (8 * i);
+                }
             }
         }
+        const uint64_t bal = __ballot(ch);
+        if (!((bal >> (lane & ~(VR - 1))) & ((1u << VR) - 1u))) continue;
+        if (ly >= th || c0 >= tw) continue;
         const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + c0;
-        const bool al16 = (((uintptr_t)im.mx | (uintptr_t)im.mn | (uintptr_t)im.opitch) & 15u) == 0;
-        if (al16 && (sel[0] & sel[1] & sel[2] & sel[3]) == 0xFu) {
-            // a whole 16-byte chunk: one store per mask, 16 rows per wave store
+        if (al16 && c0 + 16 <= tw) {
             *reinterpret_cast<uint4*>(im.mx + o) = make_uint4(mxw[0], mxw[1], mxw[2], mxw[3]);
             *reinterpret_cast<uint4*>(im.mn + o) = make_uint4(mnw[0], mnw[1], mnw[2], mnw[3]);
         } else {
 #pragma unroll
             for (int d = 0; d < 4; ++d) {
-                if (!sel[d]) continue;
-                store_quad(im.mx + o + 4 * d, mxw[d], sel[d], al_out);
-                store_quad(im.mn + o + 4 * d, mnw[d], sel[d], al_out);
+                const int nv = tw - c0 - 4 * d;
+                if (nv <= 0) break;
+                const uint32_t sel = nv >= 4 ? 0xFu : ((1u << nv) - 1u);
+                store_quad(im.mx + o + 4 * d, mxw[d], sel, al_out);
+                store_quad(im.mn + o + 4 * d, mnw[d], sel, al_out);
             }
         }
     }
