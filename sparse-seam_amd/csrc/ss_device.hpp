@@ -17,13 +17,14 @@
 //                   becomes its block-local root (dense arrays initialised);
 //                   b) lock-free min-root CAS union with path halving over the
 //                   dense ids for the seams between blocks, one thread per
-//                   seam position.
+//                   pixel of a tile edge on a block boundary.
 //   4. resolve      full compression and OR of the local evidence into roots.
 //   5. mark and     one wave per tile holding a fragment that may still change:
 //      write-back   fragments whose resolved evidence differs from the local
 //                   evidence are found; if any, the tile streams its 1-byte
 //                   per-pixel record plane (written in step 1 only by such
-//                   tiles) and rewrites only the pixels of changed fragments.
+//                   tiles: fragment ids, and the final masks of the other
+//                   pixels) and rewrites every tile row holding a changed pixel.
 //
 // Everything is exact integer work; no MFMA (nothing here is a contraction).
 #pragma once
@@ -58,6 +59,12 @@ struct ImgDev {
     int64_t tile_begin;
     int64_t block_begin;
     int32_t blocks_x, blocks_y;
+    // seam-merge units: the right edges of the last tile column of every
+    // merge-block column but the last (seam_cols of them, tiles_y tiles
+    // each), then the bottom edges of the last tile row of every merge-block
+    // row but the last (seam_rows x tiles_x)
+    int64_t seam_begin;
+    int32_t seam_cols, seam_rows;
 };
 
 struct Images {
@@ -111,8 +118,10 @@ struct Ws {
                          //   4 block root
     uint32_t* tile_img;  // [tiles] image index of each tile (batches only)
     uint32_t* block_img; // [nblk] image index of each merge block (batches only)
+    uint32_t* seam_img;  // [nseam] image index of each seam-merge unit (batches only)
     int64_t ntiles;
     int64_t nblk;        // merge blocks
+    int64_t nseam;       // seam-merge units (one tile edge each)
     uint32_t nblocks;    // scan blocks = ceil(ntiles / SCAN_CHUNK)
     int tile;            // T
     int threads;         // threads per tile

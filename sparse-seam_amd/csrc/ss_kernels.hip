@@ -27,13 +27,16 @@ __global__ __launch_bounds__(NT) void k_tile_stage(Images ims, Ws ws) {
     tile_stage<T, NT>(sm, im, gtile, ws);
 }
 
-// Per-tile and per-merge-block image index of a batch: one workgroup per image.
+// Per-tile, per-merge-block and per-seam-unit image index of a batch: one
+// workgroup per image.
 __global__ __launch_bounds__(256) void k_tile_map(Images ims, Ws ws) {
     const ImgDev im = ims.table[blockIdx.x];
     const int64_t n = (int64_t)im.tiles_x * im.tiles_y;
     for (int64_t t = threadIdx.x; t < n; t += blockDim.x) ws.tile_img[im.tile_begin + t] = blockIdx.x;
     const int64_t nb = (int64_t)im.blocks_x * im.blocks_y;
     for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) ws.block_img[im.block_begin + b] = blockIdx.x;
+    const int64_t ns = (int64_t)im.seam_cols * im.tiles_y + (int64_t)im.seam_rows * im.tiles_x;
+    for (int64_t u = threadIdx.x; u < ns; u += blockDim.x) ws.seam_img[im.seam_begin + u] = blockIdx.x;
 }
 
 // ---- compaction: exclusive scan of the per-tile counts --------------------
@@ -367,95 +370,89 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
 
 // ---- seam merge -----------------------------------------------------------
 
-// One thread per (tile, k): the tile's right-column pixel k against the three
-// pixels right of it, and its bottom-row pixel k against the three below it.
-// Together these visit every equal pixel pair that straddles a tile seam.
-// A plateau crossing a seam yields the same fragment pair at many consecutive
-// k, so a pair already present in the thread or in the neighbouring lane (the
-// previous k of the same tile edge) is not united again.
+// One thread per (seam unit, k).  A unit is a tile edge on a merge-block
+// boundary: the right edge of a tile in the last tile column of a block
+// (pixel k of its right column against the three pixels right of it), or the
+// bottom edge of a tile in the last tile row of a block (pixel k of its bottom
+// row against the three below it).  Every equal pixel pair that straddles two
+// blocks is among these: a pair across a block column is on a right edge, a
+// pair across a block row on a bottom edge, and a diagonal pair over a tile
+// corner is both the up-right contact of one tile's right edge and the
+// down-left contact of the bottom edge of the tile above-right of it (resp.
+// the down-right contact of both edges of one tile).  Pairs inside one block
+// were united by k_block_merge.  A plateau crossing a seam yields the same
+// fragment pair at many consecutive k, so a pair already present in the
+// thread or in the neighbouring lane (the previous k of the same edge) is not
+// united again.
 template <int T, bool MULTI>
 __global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t gtile = g / T;
+    const int64_t unit = g / T;
     const int k = (int)(g % T);
     const int lane = threadIdx.x & 63;
     uint32_t retries = 0;
     bool bad = false;
     constexpr uint32_t NONE = 0xFFFFFFFFu;
-    uint32_t ra = NONE, rb[3] = {NONE, NONE, NONE};   // right edge: fa and up to 3 fb
-    uint32_t ba = NONE, bb[3] = {NONE, NONE, NONE};   // bottom edge
-    if (gtile < ws.ntiles) {
-        const ImgDev im = locate<MULTI>(ims, ws.tile_img, gtile);
-        const uint32_t lt = (uint32_t)(gtile - im.tile_begin);
-        const int tx = (int)(lt % (uint32_t)im.tiles_x), ty = (int)(lt / (uint32_t)im.tiles_x);
-        // pairs inside one merge block were united by k_block_merge
-        const int bxm = tx % MERGE_BT, bym = ty % MERGE_BT;
-        const bool right_cross = bxm == MERGE_BT - 1 || (k == 0 && bym == 0) || (k == T - 1 && bym == MERGE_BT - 1);
-        const bool bottom_cross = bym == MERGE_BT - 1 || (k == 0 && bxm == 0) || (k == T - 1 && bxm == MERGE_BT - 1);
-        const uint32_t cnt = (right_cross || bottom_cross) ? (ws.counts[gtile] & COUNT_MASK) : 0u;
+    uint32_t fa = NONE, fb[3] = {NONE, NONE, NONE};
+    if (unit < ws.nseam) {
+        const ImgDev im = MULTI ? ims.table[ws.seam_img[unit]] : ims.one;
+        const uint32_t u = (uint32_t)(unit - im.seam_begin);
+        const uint32_t nv = (uint32_t)im.seam_cols * (uint32_t)im.tiles_y;
+        const bool right = u < nv;
+        // consecutive units lie on different block seams: neighbouring edges
+        // of one seam mostly join the same plateau, and taking them at the
+        // same time would only make their CAS collide
+        int tx, ty;
+        if (right) {
+            tx = MERGE_BT * (int)(u % (uint32_t)im.seam_cols) + MERGE_BT - 1;
+            ty = (int)(u / (uint32_t)im.seam_cols);
+        } else {
+            const uint32_t v = u - nv;
+            ty = MERGE_BT * (int)(v % (uint32_t)im.seam_rows) + MERGE_BT - 1;
+            tx = (int)(v / (uint32_t)im.seam_rows);
+        }
+        // by construction the tile right of (below) the edge exists
+        const int64_t gtile = im.tile_begin + (int64_t)ty * im.tiles_x + tx;
+        const uint32_t cnt = ws.counts[gtile] & COUNT_MASK;
         const uint8_t* eq = ws.eq + gtile * 2 * T;
         const uint8_t* rec = ws.rec + gtile * 4 * T;
-        const uint32_t base = cnt ? ws.base[gtile] : 0u;
-        if (cnt && right_cross && tx + 1 < im.tiles_x) {
-            const uint32_t e = eq[k];
-            if (e) {
-                const uint32_t ja = rec[3 * T + k];
-                if (ja < cnt) ra = base + ja; else bad = true;
-                const int y = ty * T + k;
+        const uint32_t e = cnt ? (uint32_t)eq[right ? k : T + k] : 0u;
+        if (e) {
+            const uint32_t base = ws.base[gtile];
+            const uint32_t ja = rec[right ? 3 * T + k : T + k];
+            if (ja < cnt) fa = base + ja; else bad = true;
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    if (!(e & (1u << d)) || ra == NONE) continue;
-                    const int ny = y + d - 1;
-                    const int nty = ny / T;
-                    if (bxm != MERGE_BT - 1 && nty / MERGE_BT == ty / MERGE_BT) continue;
-                    const int64_t nt = im.tile_begin + (int64_t)nty * im.tiles_x + tx + 1;
-                    const uint32_t jb = ws.rec[nt * 4 * T + 2 * T + (ny - nty * T)];
-                    if (jb >= (ws.counts[nt] & COUNT_MASK)) { bad = true; continue; }
-                    rb[d] = ws.base[nt] + jb;
+            for (int d = 0; d < 3; ++d) {
+                if (!(e & (1u << d)) || fa == NONE) continue;
+                int64_t nt;
+                uint32_t slot;
+                if (right) {
+                    const int ny = ty * T + k + d - 1, nty = ny / T;
+                    nt = im.tile_begin + (int64_t)nty * im.tiles_x + tx + 1;
+                    slot = 2 * T + (ny - nty * T);   // left column of the tile to the right
+                } else {
+                    const int nx = tx * T + k + d - 1, ntx = nx / T;
+                    nt = im.tile_begin + (int64_t)(ty + 1) * im.tiles_x + ntx;
+                    slot = nx - ntx * T;             // top row of the tile below
                 }
-            }
-        }
-        if (cnt && bottom_cross && ty + 1 < im.tiles_y) {
-            const uint32_t e = eq[T + k];
-            if (e) {
-                const uint32_t ja = rec[T + k];
-                if (ja < cnt) ba = base + ja; else bad = true;
-                const int x = tx * T + k;
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    if (!(e & (1u << d)) || ba == NONE) continue;
-                    const int nx = x + d - 1;
-                    const int ntx = nx / T;
-                    if (bym != MERGE_BT - 1 && ntx / MERGE_BT == tx / MERGE_BT) continue;
-                    const int64_t nt = im.tile_begin + (int64_t)(ty + 1) * im.tiles_x + ntx;
-                    const uint32_t jb = ws.rec[nt * 4 * T + (nx - ntx * T)];
-                    if (jb >= (ws.counts[nt] & COUNT_MASK)) { bad = true; continue; }
-                    bb[d] = ws.base[nt] + jb;
-                }
+                const uint32_t jb = ws.rec[nt * 4 * T + slot];
+                if (jb >= (ws.counts[nt] & COUNT_MASK)) { bad = true; continue; }
+                fb[d] = ws.base[nt] + jb;
             }
         }
     }
-    // pairs of the previous k (lane - 1) on the same kind of edge
-    const uint32_t pra = __shfl_up(ra, 1, 64), pba = __shfl_up(ba, 1, 64);
-    uint32_t prb[3], pbb[3];
+    // pairs of the previous k (lane - 1) on the same edge
+    const uint32_t pfa = __shfl_up(fa, 1, 64);
+    uint32_t pfb[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        prb[d] = __shfl_up(rb[d], 1, 64);
-        pbb[d] = __shfl_up(bb[d], 1, 64);
-    }
+    for (int d = 0; d < 3; ++d) pfb[d] = __shfl_up(fb[d], 1, 64);
     const bool has_prev = lane > 0 && k > 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        if (rb[d] != NONE) {
-            bool dup = (d > 0 && rb[d] == rb[d - 1]) || (d > 1 && rb[d] == rb[d - 2]);
-            if (has_prev && pra == ra) dup = dup || rb[d] == prb[0] || rb[d] == prb[1] || rb[d] == prb[2];
-            if (!dup) gunion(ws.dparent, ra, rb[d], retries);
-        }
-        if (bb[d] != NONE) {
-            bool dup = (d > 0 && bb[d] == bb[d - 1]) || (d > 1 && bb[d] == bb[d - 2]);
-            if (has_prev && pba == ba) dup = dup || bb[d] == pbb[0] || bb[d] == pbb[1] || bb[d] == pbb[2];
-            if (!dup) gunion(ws.dparent, ba, bb[d], retries);
-        }
+        if (fb[d] == NONE) continue;
+        bool dup = (d > 0 && fb[d] == fb[d - 1]) || (d > 1 && fb[d] == fb[d - 2]);
+        if (has_prev && pfa == fa) dup = dup || fb[d] == pfb[0] || fb[d] == pfb[1] || fb[d] == pfb[2];
+        if (!dup) gunion(ws.dparent, fa, fb[d], retries);
     }
     const uint32_t r = wave_sum(retries);
     if (lane == 0 && r) atomicAdd((unsigned long long*)&ws.hdr->retries, (unsigned long long)r);
@@ -551,9 +548,11 @@ static void launch_writeback(const Images& ims, const Ws& ws, int grid_cap, hipS
 template <int T, bool MULTI>
 static void launch_merge(const Images& ims, const Ws& ws, hipStream_t s) {
     k_block_merge<T, MULTI><<<(unsigned)ws.nblk, MERGE_THREADS, 0, s>>>(ims, ws);
-    const int64_t merge_threads = ws.ntiles * T;
-    const unsigned merge_grid = (unsigned)((merge_threads + 255) / 256);
-    k_seam_merge<T, MULTI><<<merge_grid, 256, 0, s>>>(ims, ws);
+    const int64_t merge_threads = ws.nseam * T;
+    if (merge_threads > 0) {
+        const unsigned merge_grid = (unsigned)((merge_threads + 255) / 256);
+        k_seam_merge<T, MULTI><<<merge_grid, 256, 0, s>>>(ims, ws);
+    }
 }
 
 #define SSX_DISPATCH(T_, NT_, CALL)                                        \
