@@ -54,6 +54,10 @@ __device__ __forceinline__ uint32_t q_above(uint32_t st, uint32_t r) {
     return (((st >> Q_EUL) & r) | (((st >> Q_EU) & r) << 1) | (((st >> Q_EUR) & r) << 2)) & 0x3Fu;
 }
 
+// Nibble -> bytes: bit m to bit m of byte m (diag4), to bit 0 of byte m (spread4).
+__device__ __forceinline__ uint32_t diag4(uint32_t n) { return (n * 0x01010101u) & 0x08040201u; }
+__device__ __forceinline__ uint32_t spread4(uint32_t n) { return (n * 0x00204081u) & 0x01010101u; }
+
 // SWAR helpers on four bytes (one per pixel of a quad).
 // Top bit of every byte -> 4-bit mask.
 __device__ __forceinline__ uint32_t movemask4(uint32_t w) {
@@ -395,24 +399,39 @@ __device__ __forceinline__ uint32_t classify_quad(TileSmem<T>& sm, const RowW& a
         const uint32_t p0 = ly * T + lx0;
         const uint32_t hd = q_heads(s);
         const uint32_t el4 = (s >> Q_EL) & 0xFu;
+        // All four pixels at once, one byte each.  Byte m of P: the contacts
+        // of pixel m (bits m, m + 1, m + 2 = up-left, up, up-right, in the
+        // 6-bit space of q_above); byte m of R: their OR from pixel m to the
+        // end of its run, i.e. at a head the run's contacts.  Bytes stay below
+        // 64, so the per-byte arithmetic below never carries across bytes.
+        const uint32_t P = diag4((s >> Q_EUL) & 0xFu) | diag4((s >> Q_EU) & 0xFu) << 1 |
+                           diag4((s >> Q_EUR) & 0xFu) << 2;
+        const uint32_t m1 = spread4(el4 >> 1) * 0xFFu;   // byte m: pixel m + 1 continues the run
+        uint32_t R = P | ((P >> 8) & m1);
+        R |= (R >> 16) & (m1 & (m1 >> 8));
+        // 0x80 per byte: the run has no contact / two contact segments or more
+        const uint32_t z80 = ~(((R & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | R) & 0x80808080u;
+        const uint32_t sg = R & ~(R << 1);
+        const uint32_t s2 = sg & ((sg | 0x80808080u) - 0x01010101u);
+        const uint32_t m80 = (((s2 & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | s2) & 0x80808080u;
+        const uint32_t left = el4 & 1u;   // pixel 0 continues a run of the quad to the left
+        const uint32_t r0 = R & 0xFFu;
+        // union pass: further segments, or a left neighbour the link does not
+        // reach (see quad_label); roots: heads linked to themselves
+        const uint32_t un = (movemask4(m80) | ((left && r0 && !(r0 & 3u)) ? 1u : 0u)) & hd;
+        const uint32_t rt = movemask4(z80) & hd & ~left;
+        // a head links to its first contact, else left, else itself; the other
+        // pixels of a run to its head (singletons: an unread dummy)
         uint32_t lab4[4];
         uint32_t head = p0;
-        uint32_t un = 0, rt = 0;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if ((hd >> i) & 1u) {
-                head = p0 + i;
-                const uint32_t ab = q_above(s, q_run(s, i));
-                const bool left = i == 0 && (el4 & 1u);
-                lab4[i] = ab ? (p0 - T + __builtin_ctz(ab) - 1u) : (left ? p0 - 1u : p0 + i);
-                // contact segments beyond the first, or a left neighbour the
-                // link does not reach (see quad_label)
-                const uint32_t seg = ab & ~(ab << 1);
-                if ((seg & (seg - 1u)) || (left && ab && !(ab & 3u))) un |= 1u << i;
-                if (!ab && !left) rt |= 1u << i;
-            } else {
-                lab4[i] = head;   // run pixel (or an unread dummy)
-            }
+        for (int m = 0; m < 4; ++m) {
+            const uint32_t rm = (R >> (8 * m)) & 0xFFu;
+            const uint32_t self = (m == 0 && left) ? p0 - 1u : p0 + m;
+            const uint32_t link = rm ? p0 - T - 1u + __builtin_ctz(rm | 0x40u) : self;
+            const bool is_head = (hd >> m) & 1u;
+            head = is_head ? p0 + m : head;
+            lab4[m] = is_head ? link : head;
         }
         st = s | un << Q_UN;
         rtb = (rt * 0x00204081u) & B_RT;
