@@ -158,27 +158,13 @@ __device__ __forceinline__ uint32_t lds_load(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Read-only LDS find (plain labels).
 __device__ __forceinline__ uint32_t lfind(const uint32_t* lab, uint32_t x) {
     uint32_t r = lds_load(lab + x);
     for (;;) {
         const uint32_t n = lds_load(lab + r);
         if (n == r) return r;
         r = n;
-    }
-}
-
-__device__ __forceinline__ void lunion(uint32_t* lab, uint32_t a, uint32_t b) {
-    for (;;) {
-        a = lfind(lab, a);
-        b = lfind(lab, b);
-        if (a == b) return;
-        const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
-        uint32_t expected = hi;
-        if (__hip_atomic_compare_exchange_strong(lab + hi, &expected, lo, __ATOMIC_RELAXED,
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
-            return;
-        a = hi;
-        b = lo;
     }
 }
 
