@@ -98,6 +98,25 @@ ss_status ss_read_stats(const void* d_ws, size_t ws_bytes, ss_stats* out, hipStr
 ss_status ss_read_tile_counts(const void* d_ws, size_t ws_bytes, uint32_t* out, size_t count,
                               hipStream_t stream);
 
+/* Seam-sparsity counters of one image at tile size cfg->tile (threads are
+ * fixed at 256; NULL = the process default tile), the quantities the
+ * reference's seam profiler reports per image
+ * (harness/seam_sparsity_profiler/gpu_seam_sparsity_profiler_v0_1.cu:133-294,
+ * CSV columns :391-396; fractions and per-megapixel rates are ratios of these).
+ * A diagnostic, not part of the extrema path: it stages and labels every tile
+ * but writes no masks.  Uses the workspace of ss_workspace_size; synchronous. */
+typedef struct ss_seam_profile {
+    uint64_t fragments;       /* tile-local zones, singletons included */
+    uint64_t seam_fragments;  /* zones with an equal 8-neighbour in another tile */
+    uint64_t seam_pixels;     /* pixels of those zones */
+    uint64_t equal_edges;     /* equal 8-neighbour pairs inside the image, each once */
+    uint64_t seam_edges;      /* those pairs whose two pixels lie in different tiles */
+} ss_seam_profile;
+
+ss_status ss_seam_profile_u8(const uint8_t* d_img, size_t pitch, int width, int height,
+                             const ss_config* cfg, void* d_ws, size_t ws_bytes, hipStream_t stream,
+                             ss_seam_profile* out);
+
 /* ------------------------------------------------------------------------ */
 /* Batched form: many images of uneven sizes in one pipeline.               */
 /* No reference counterpart: the reference loops over images, one          */

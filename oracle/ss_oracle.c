@@ -30,6 +30,11 @@
  * ssor_quantize restates quantize_grayscale_in_place
  * (harness/quantization_and_scaling/gpu_common_quant_v0_7.cu:479-495).
  *
+ * ssor_seam_profile restates the seam-sparsity counters of
+ * harness/seam_sparsity_profiler/gpu_seam_sparsity_profiler_v0_1.cu
+ * (139-171 tile-local unions, 173-183 compression, 185-249 equal forward
+ * edges and seam marks, 251-294 fragment and pixel counts).
+ *
  * Every route is pinned by the golden fixtures and by agreement with the
  * brute-force closed form on random images (tests/test_oracle.py).
  */
@@ -450,4 +455,82 @@ uint64_t ssor_count_mismatch(const uint8_t* a, const uint8_t* b, size_t n)
     uint64_t m = 0;
     for (size_t i = 0; i < n; ++i) m += ((a[i] != 0) != (b[i] != 0));
     return m;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Seam-sparsity counters (gpu_seam_sparsity_profiler_v0_1.cu:133-294).     */
+/* ------------------------------------------------------------------------ */
+
+/* out[0] tile-local zones ("fragments", singletons included), out[1] zones
+ * with an equal 8-neighbour in another tile, out[2] pixels of those zones,
+ * out[3] equal 8-neighbour pairs (each once: right, down-left, down,
+ * down-right, both pixels in the image), out[4] those pairs whose pixels lie
+ * in different tiles. */
+int ssor_seam_profile(const uint8_t* img, int w, int h, int T, uint64_t* out)
+{
+    if (!img || !out || w <= 0 || h <= 0 || T <= 0) return SSOR_ERR_ARG;
+    const int64_t n = (int64_t)w * h;
+    int32_t* parent = (int32_t*)malloc((size_t)n * sizeof(int32_t));
+    uint8_t* seam = (uint8_t*)calloc((size_t)n, 1);
+    if (!parent || !seam) {
+        free(parent);
+        free(seam);
+        return SSOR_ERR_NOMEM;
+    }
+    for (int64_t i = 0; i < n; ++i) parent[i] = (int32_t)i;
+    /* 139-171: unions of the equal left / up / up-left / up-right pairs that
+     * stay inside one tile */
+    for (int y = 0; y < h; ++y) {
+        for (int x = 0; x < w; ++x) {
+            const int64_t i = (int64_t)y * w + x;
+            const uint8_t v = img[i];
+            const int tx = x / T, ty = y / T;
+            if (x > 0 && (x - 1) / T == tx && img[i - 1] == v) ss_union(parent, (int32_t)i, (int32_t)(i - 1));
+            if (y > 0 && (y - 1) / T == ty) {
+                if (img[i - w] == v) ss_union(parent, (int32_t)i, (int32_t)(i - w));
+                if (x > 0 && (x - 1) / T == tx && img[i - w - 1] == v)
+                    ss_union(parent, (int32_t)i, (int32_t)(i - w - 1));
+                if (x + 1 < w && (x + 1) / T == tx && img[i - w + 1] == v)
+                    ss_union(parent, (int32_t)i, (int32_t)(i - w + 1));
+            }
+        }
+    }
+    for (int64_t i = 0; i < n; ++i) parent[i] = ss_find(parent, (int32_t)i);
+    /* 185-249: equal forward pairs; a pair across tiles marks both zones */
+    uint64_t equal = 0, seam_edges = 0;
+    static const int dx[4] = {1, -1, 0, 1}, dy[4] = {0, 1, 1, 1};
+    for (int y = 0; y < h; ++y) {
+        for (int x = 0; x < w; ++x) {
+            const int64_t i = (int64_t)y * w + x;
+            for (int d = 0; d < 4; ++d) {
+                const int nx = x + dx[d], ny = y + dy[d];
+                if (nx < 0 || nx >= w || ny >= h) continue;
+                const int64_t j = (int64_t)ny * w + nx;
+                if (img[j] != img[i]) continue;
+                ++equal;
+                if (nx / T != x / T || ny / T != y / T) {
+                    ++seam_edges;
+                    seam[parent[i]] = 1;
+                    seam[parent[j]] = 1;
+                }
+            }
+        }
+    }
+    /* 251-294: zones are the roots; seam zones and their pixels */
+    uint64_t zones = 0, seam_zones = 0, seam_px = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (parent[i] == (int32_t)i) {
+            ++zones;
+            seam_zones += seam[i];
+        }
+        seam_px += seam[parent[i]];
+    }
+    out[0] = zones;
+    out[1] = seam_zones;
+    out[2] = seam_px;
+    out[3] = equal;
+    out[4] = seam_edges;
+    free(parent);
+    free(seam);
+    return SSOR_OK;
 }

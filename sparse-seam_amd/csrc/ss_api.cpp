@@ -145,6 +145,34 @@ ss_status ss_read_tile_counts(const void* d_ws, size_t ws_bytes, uint32_t* out, 
 
 // ---- batch ----------------------------------------------------------------
 
+ss_status ss_seam_profile_u8(const uint8_t* d_img, size_t pitch, int width, int height,
+                             const ss_config* cfg, void* d_ws, size_t ws_bytes, hipStream_t stream,
+                             ss_seam_profile* out) {
+    if (!d_img || !out || width <= 0 || height <= 0 || pitch < (size_t)width) return SS_ERR_ARG;
+    ss_config c = resolve_config(cfg);
+    c.threads = 256;
+    if (!config_valid(c.tile, c.threads)) return SS_ERR_ARG;
+    const int64_t nt = tiles_of(width, height, c.tile);
+    if (nt * (4 * c.tile - 4) > (int64_t)0xFFFFFFF0u) return SS_ERR_ARG;
+    Images ims{};
+    ims.one = make_imgdev(d_img, nullptr, nullptr, (int64_t)pitch, (int64_t)width, width, height, c.tile, 0, 0, 0);
+    ims.count = 1;
+    const Layout L = make_layout(nt, blocks_of(ims.one), seams_of(ims.one), c.tile, 1);
+    if (!d_ws || ws_bytes < L.bytes || !aligned256(d_ws)) return SS_ERR_WORKSPACE;
+    const Ws ws = make_ws(d_ws, L, c.threads);
+    if (launch_profile(ims, ws, c.tile, stream) != hipSuccess) return SS_ERR_HIP;
+    uint64_t v[5];
+    if (hipMemcpyAsync(v, &ws.hdr->profile[0], sizeof(v), hipMemcpyDeviceToHost, stream) != hipSuccess)
+        return SS_ERR_HIP;
+    if (hipStreamSynchronize(stream) != hipSuccess) return SS_ERR_HIP;
+    out->fragments = v[0];
+    out->seam_fragments = v[1];
+    out->seam_pixels = v[2];
+    out->equal_edges = v[3];
+    out->seam_edges = v[4];
+    return SS_OK;
+}
+
 static ss_status batch_prepare(const ss_image_desc* images, int n, const ss_config& c,
                                std::vector<ImgDev>& table, Layout& L) {
     if (!images || n <= 0) return SS_ERR_ARG;

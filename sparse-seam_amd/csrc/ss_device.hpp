@@ -85,7 +85,8 @@ struct WsHeader {
     uint32_t nblocks;      // scan blocks
     uint64_t retries;      // failed CAS in the seam union-find
     uint64_t tiles;
-    uint32_t pad[64 - 10 - 2];
+    uint64_t profile[5];   // seam-sparsity counters (ss_seam_profile_u8 only)
+    uint32_t pad[64 - 10 - 2 - 10];
 };
 static_assert(sizeof(WsHeader) == 256, "header must be 256 bytes");
 

@@ -142,4 +142,8 @@ ProfileHook& profile_hook();
 // Enqueues the whole pipeline on `s`.
 hipError_t launch_pipeline(const Images& ims, const Ws& ws, int T, int NT, int cus, hipStream_t s);
 
+// Enqueues the seam-sparsity profile of one image (counters zeroed first, in
+// WsHeader::profile).
+hipError_t launch_profile(const Images& ims, const Ws& ws, int T, hipStream_t s);
+
 }  // namespace ssx

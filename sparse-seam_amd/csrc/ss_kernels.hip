@@ -529,6 +529,34 @@ __global__ __launch_bounds__(256) void k_writeback(Images ims, Ws ws) {
     if (threadIdx.x == 0 && s_dirty) atomicAdd(&ws.hdr->dirty_count, s_dirty);
 }
 
+// ---- seam-sparsity profile (diagnostic) ------------------------------------
+
+// One workgroup per tile; per-tile sums in LDS, one 64-bit atomic per counter
+// and workgroup into the workspace header.
+template <int T>
+__global__ __launch_bounds__(256) void k_seam_profile(Images ims, Ws ws) {
+    __shared__ TileSmem<T> sm;
+    __shared__ uint32_t c[5];
+    if (threadIdx.x < 5) c[threadIdx.x] = 0;   // ordered by the barrier after the tile load
+    tile_profile<T, 256>(sm, ims.one, (int64_t)blockIdx.x, c);
+    __syncthreads();
+    if (threadIdx.x < 5 && c[threadIdx.x])
+        atomicAdd((unsigned long long*)&ws.hdr->profile[threadIdx.x], (unsigned long long)c[threadIdx.x]);
+}
+
+hipError_t launch_profile(const Images& ims, const Ws& ws, int T, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(&ws.hdr->profile[0], 0, sizeof(ws.hdr->profile), s);
+    if (e != hipSuccess) return e;
+    const unsigned grid = (unsigned)ws.ntiles;
+    switch (T) {
+    case 16: k_seam_profile<16><<<grid, 256, 0, s>>>(ims, ws); break;
+    case 32: k_seam_profile<32><<<grid, 256, 0, s>>>(ims, ws); break;
+    case 64: k_seam_profile<64><<<grid, 256, 0, s>>>(ims, ws); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 // ---- host sequencer -------------------------------------------------------
 
 // One workgroup per tile over the combined tile space of all images: for a

@@ -684,6 +684,90 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
     if (threadIdx.x == 0) ws.counts[gtile] = min(count, (uint32_t)S) | tflags | (sm.err << COUNT_ERR_SHIFT);
 }
 
+// Seam-sparsity counters of one tile (diagnostic, not on the extrema path),
+// the quantities of the reference profiler
+// (gpu_seam_sparsity_profiler_v0_1.cu:133-294): c[0] tile-local zones
+// (singletons included), c[1] zones with an equal 8-neighbour in another tile,
+// c[2] their pixels, c[3] equal 8-neighbour pairs with both pixels in the
+// image, each counted once from its upper / left pixel (right, down-left,
+// down, down-right), c[4] those pairs whose pixels lie in different tiles.
+// Reuses the tile stage's staging, classification, seam tests and labelling.
+template <int T, int NT>
+__device__ __forceinline__ void tile_profile(TileSmem<T>& sm, const ImgDev& im, int64_t gtile, uint32_t* c) {
+    constexpr int PER = Quads<T, NT>::PER;
+    const uint32_t lt = (uint32_t)(gtile - im.tile_begin);
+    const int tx = (int)(lt % (uint32_t)im.tiles_x), ty = (int)(lt / (uint32_t)im.tiles_x);
+    const int x0 = tx * T, y0 = ty * T;
+    const int tw = min(T, im.w - x0), th = min(T, im.h - y0);
+    if (threadIdx.x == 0) sm.zflags = 0;
+    tile_load<T, NT>(sm, im, x0, y0, tw, th);
+    __syncthreads();
+    const uint32_t v0 = sm.img[TileSmem<T>::P + 4];
+    uint32_t st[PER], sb[PER];
+    const bool mine_eq = quad_init<T, NT>(sm, tw, th, st, sb, v0);
+    const uint32_t seam_bits = perimeter_pass<T, NT>(sm, im, x0, y0, tw, th);
+    const bool uniform = __syncthreads_and(mine_eq) != 0;
+    uint32_t zones = 0, seam_zones = 0, seam_px = 0, equal = 0, seam_edges = 0;
+    // equal forward pairs from the staged rows (halo included; out-of-image
+    // neighbours masked)
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        int ly, lx0;
+        if (!Quads<T, NT>::at(k, ly, lx0) || ly >= th || lx0 >= tw) continue;
+        const RowW b = row_at<T>(sm, ly, lx0), d = row_at<T>(sm, ly + 1, lx0);
+        const uint32_t val = first_bytes80(tw - lx0);
+        const int gx = x0 + lx0;
+        const uint32_t right_img = first_bytes80(im.w - gx - 1), left_img = gx > 0 ? 0x80808080u : 0x80808000u;
+        const uint32_t down_in_img = (y0 + ly + 1 < im.h) ? 0x80808080u : 0u;
+        const uint32_t right_tile = first_bytes80(tw - lx0 - 1), left_tile = lx0 > 0 ? 0x80808080u : 0x80808000u;
+        const uint32_t down_tile = (ly + 1 < th) ? 0x80808080u : 0u;
+        const uint32_t eR = ~ne_top(b.r, b.w) & val & right_img;
+        const uint32_t eD = ~ne_top(d.w, b.w) & val & down_in_img;
+        const uint32_t eDL = ~ne_top(d.l, b.w) & val & left_img & down_in_img;
+        const uint32_t eDR = ~ne_top(d.r, b.w) & val & right_img & down_in_img;
+        equal += __builtin_popcount((eR | (eD >> 1) | (eDL >> 2) | (eDR >> 3)) & 0xF0F0F0F0u);
+        seam_edges += __builtin_popcount(((eR & ~right_tile) | ((eD & ~down_tile) >> 1) |
+                                          ((eDL & ~(left_tile & down_tile)) >> 2) |
+                                          ((eDR & ~(right_tile & down_tile)) >> 3)) & 0xF0F0F0F0u);
+    }
+    if (uniform) {
+        const bool any_seam = __syncthreads_or(seam_bits != 0) != 0;
+        if (threadIdx.x == 0) {
+            zones = 1;
+            seam_zones = any_seam ? 1u : 0u;
+            seam_px = any_seam ? (uint32_t)(tw * th) : 0u;
+        }
+    } else {
+        quad_label<T, NT>(sm, st, sb, tw, th, seam_bits);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            int ly, lx0;
+            Quads<T, NT>::at(k, ly, lx0);
+            zones += __builtin_popcount(sb[k] & B_SGL);   // singletons: never on a seam
+            const uint32_t p0 = ly * T + lx0;
+            uint32_t hd = q_heads(st[k]);
+            while (hd) {
+                const int i = __builtin_ctz(hd);
+                hd &= hd - 1u;
+                const uint32_t w = lds_load(&sm.lab[p0 + i]);
+                const uint32_t rw = (w & LMASK) == p0 + i ? w : lds_load(&sm.lab[w & LMASK]);
+                const bool seam = (rw >> FLAG_SHIFT) & F_SEAM;
+                if ((w & LMASK) == p0 + i) {
+                    ++zones;
+                    seam_zones += seam ? 1u : 0u;
+                }
+                if (seam) seam_px += __builtin_popcount(q_run(st[k], i));
+            }
+        }
+    }
+    const uint32_t vals[5] = {zones, seam_zones, seam_px, equal, seam_edges};
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const uint32_t v = wave_sum(vals[q]);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&c[q], v);
+    }
+}
+
 // Mark and write-back of one tile by one wave (ss_device.hpp, step 5).
 // cw is the tile's counts word; only a tile with an open fragment, or a
 // uniform tile (whose one fragment may be open), can change.  Lanes resolve

@@ -33,6 +33,12 @@ class Stats(ctypes.Structure):
                 ("tiles", ctypes.c_uint64)]
 
 
+class SeamProfile(ctypes.Structure):
+    _fields_ = [("fragments", ctypes.c_uint64), ("seam_fragments", ctypes.c_uint64),
+                ("seam_pixels", ctypes.c_uint64), ("equal_edges", ctypes.c_uint64),
+                ("seam_edges", ctypes.c_uint64)]
+
+
 class ImageDesc(ctypes.Structure):
     _fields_ = [("img", ctypes.c_void_p), ("max", ctypes.c_void_p), ("min", ctypes.c_void_p),
                 ("pitch", ctypes.c_uint64), ("out_pitch", ctypes.c_uint64),
@@ -57,6 +63,7 @@ def _lib():
     lib.ss_extrema_u8.argtypes = [vp, sz, i, i, vp, vp, sz, vp, sz, cfgp, vp]
     lib.ss_read_stats.argtypes = [vp, sz, ctypes.POINTER(Stats), vp]
     lib.ss_read_tile_counts.argtypes = [vp, sz, ctypes.POINTER(ctypes.c_uint32), sz, vp]
+    lib.ss_seam_profile_u8.argtypes = [vp, sz, i, i, cfgp, vp, sz, vp, ctypes.POINTER(SeamProfile)]
     descp = ctypes.POINTER(ImageDesc)
     lib.ss_batch_workspace_size.argtypes = [descp, i, cfgp, ctypes.POINTER(sz)]
     lib.ss_extrema_batch_u8.argtypes = [descp, i, vp, sz, cfgp, vp]
@@ -157,6 +164,38 @@ def stats(ws, stream=None, check=True):
         _check(status, "ss_read_stats")
     return {"status": status, "invariant": st.invariant, "dirty_tiles": st.dirty_tiles,
             "fragments": st.fragments, "dsu_retries": st.dsu_retries, "tiles": st.tiles}
+
+
+SEAM_PROFILE_KEYS = ("fragments", "seam_fragments", "seam_pixels", "equal_edges", "seam_edges")
+
+
+def seam_profile(img, tile=None, ws=None, stream=None):
+    """Seam-sparsity counters of a 2-D uint8 CUDA tensor at tile size `tile`
+    (dict of SEAM_PROFILE_KEYS; the reference profiler's per-image counts)."""
+    import torch
+    if img.dim() != 2 or img.dtype != torch.uint8 or not img.is_cuda or img.stride(1) != 1:
+        raise ValueError("img must be a 2-D row-major uint8 CUDA tensor")
+    h, w = img.shape
+    need = workspace_size(w, h, tile, 256)
+    if ws is None or ws.numel() < need:
+        ws = new_workspace(need, img.device)
+    out = SeamProfile()
+    _check(_lib().ss_seam_profile_u8(img.data_ptr(), img.stride(0), w, h, _cfg(tile, 256), ws.data_ptr(),
+                                     ws.numel(), _stream_handle(stream), ctypes.byref(out)),
+           "ss_seam_profile_u8")
+    return {k: int(getattr(out, k)) for k in SEAM_PROFILE_KEYS}
+
+
+def seam_profile_row(counts, width, height):
+    """The reference profiler's derived columns from the counts
+    (gpu_seam_sparsity_profiler_v0_1.cu:482-517)."""
+    mp = width * height / 1e6
+    f, sf = counts["fragments"], counts["seam_fragments"]
+    return dict(counts, interior_fragments=f - sf,
+                seam_fragment_fraction=sf / f if f else 0.0,
+                seam_pixel_fraction=counts["seam_pixels"] / (width * height),
+                seam_edge_fraction=counts["seam_edges"] / counts["equal_edges"] if counts["equal_edges"] else 0.0,
+                seam_fragments_per_mpix=sf / mp, equal_seam_edges_per_mpix=counts["seam_edges"] / mp)
 
 
 def tile_counts(ws, tiles_y, tiles_x, stream=None):

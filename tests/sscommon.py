@@ -78,6 +78,8 @@ def oracle():
     lib.ssor_quantize.restype = None
     lib.ssor_count_mismatch.argtypes = [u8p, u8p, ctypes.c_size_t]
     lib.ssor_count_mismatch.restype = ctypes.c_uint64
+    lib.ssor_seam_profile.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_uint64)]
     _oracle = lib
     return lib
 
@@ -118,6 +120,19 @@ def oracle_sparse_seam(img, tile):
                                    counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)))
     assert rc == 0, rc
     return mx, mn, counts
+
+
+SEAM_PROFILE_KEYS = ("fragments", "seam_fragments", "seam_pixels", "equal_edges", "seam_edges")
+
+
+def oracle_seam_profile(img, tile):
+    """Seam-sparsity counters of the reference profiler (dict of SEAM_PROFILE_KEYS)."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    h, w = img.shape
+    out = (ctypes.c_uint64 * 5)()
+    rc = oracle().ssor_seam_profile(_p(img), w, h, tile, out)
+    assert rc == 0, rc
+    return dict(zip(SEAM_PROFILE_KEYS, (int(v) for v in out)))
 
 
 def oracle_quantize(img, levels):

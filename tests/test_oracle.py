@@ -5,7 +5,7 @@ import pytest
 from scipy import ndimage
 
 from sscommon import (golden_cases, mismatches, oracle_bruteforce, oracle_global_uf,
-                      oracle_quantize, oracle_sparse_seam)
+                      oracle_quantize, oracle_seam_profile, oracle_sparse_seam)
 
 CASES = list(golden_cases())
 EIGHT = np.ones((3, 3), dtype=bool)
@@ -187,3 +187,73 @@ def test_quantizer_closed_form():
     assert np.array_equal(q4, np.array([0, 85, 170, 255], np.uint8)[v // 64])
     q32 = oracle_quantize(v, 32)
     assert len(np.unique(q32)) == 32 and q32[0] == 0 and q32[255] == 255
+
+
+def scipy_seam_profile(img, tile):
+    """Independent closed form of the seam-sparsity counters: scipy labels of
+    every grey level inside every tile; a zone is a seam zone iff one of its
+    pixels has an equal 8-neighbour inside the image but outside the tile;
+    equal pairs are counted by shifted array comparisons."""
+    a = np.asarray(img, dtype=np.int16)
+    h, w = a.shape
+    zones = seam_zones = seam_px = 0
+    for y0 in range(0, h, tile):
+        for x0 in range(0, w, tile):
+            sub = a[y0:y0 + tile, x0:x0 + tile]
+            inside = np.zeros((h, w), bool)
+            inside[y0:y0 + sub.shape[0], x0:x0 + sub.shape[1]] = True
+            for level in np.unique(sub):
+                lab, n = ndimage.label(sub == level, structure=EIGHT)
+                zones += n
+                full = np.zeros((h, w), np.int32)
+                full[y0:y0 + sub.shape[0], x0:x0 + sub.shape[1]] = lab
+                # equal pixels outside the tile, dilated back onto the tile
+                outside_eq = (a == level) & ~inside
+                touch = ndimage.binary_dilation(outside_eq, structure=EIGHT) & (full > 0)
+                hit = np.unique(full[touch])
+                seam_zones += len(hit)
+                seam_px += int(np.isin(full, hit[hit > 0]).sum())
+    equal = seam_e = 0
+    ys, xs = np.indices((h, w))
+    for dy, dx in ((0, 1), (1, -1), (1, 0), (1, 1)):
+        y1, y2 = 0, h - dy
+        x1, x2 = max(0, -dx), w - max(0, dx)
+        p = a[y1:y2, x1:x2]
+        q = a[y1 + dy:y2 + dy, x1 + dx:x2 + dx]
+        eq = p == q
+        cross = ((ys[y1:y2, x1:x2] // tile) != ((ys[y1:y2, x1:x2] + dy) // tile)) | \
+                ((xs[y1:y2, x1:x2] // tile) != ((xs[y1:y2, x1:x2] + dx) // tile))
+        equal += int(eq.sum())
+        seam_e += int((eq & cross).sum())
+    return {"fragments": zones, "seam_fragments": seam_zones, "seam_pixels": seam_px,
+            "equal_edges": equal, "seam_edges": seam_e}
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_seam_profile_closed_form(seed):
+    rng = np.random.default_rng(300 + seed)
+    h, w = int(rng.integers(12, 80)), int(rng.integers(12, 80))
+    img = _random_image(rng, h, w, int(rng.choice([2, 3, 8, 256])), ["iid", "blocks", "ramp"][seed % 3])
+    for tile in (4, 16, 32):
+        prof = oracle_seam_profile(img, tile)
+        assert prof == scipy_seam_profile(img, tile), tile
+        # the seam zones are the per-tile seam-fragment counts of the
+        # Sparse-Seam restatement
+        assert prof["seam_fragments"] == int(oracle_sparse_seam(img, tile)[2].sum())
+
+
+def test_seam_profile_special_cases():
+    # constant image: one zone per tile, every pair equal
+    h, w, t = 37, 50, 16
+    prof = oracle_seam_profile(np.full((h, w), 9, np.uint8), t)
+    tiles = -(-h // t) * -(-w // t)
+    pairs = h * (w - 1) + (h - 1) * w + 2 * (h - 1) * (w - 1)
+    assert prof["fragments"] == tiles and prof["equal_edges"] == pairs
+    assert prof["seam_fragments"] == tiles and prof["seam_pixels"] == h * w
+    # one tile covering the image: no seams
+    prof = oracle_seam_profile(np.full((8, 8), 1, np.uint8), 16)
+    assert prof["seam_edges"] == 0 and prof["seam_fragments"] == 0
+    # checkerboard: no equal horizontal or vertical pairs, all diagonal pairs equal
+    cb = (np.indices((20, 20)).sum(0) % 2).astype(np.uint8)
+    prof = oracle_seam_profile(cb, 8)
+    assert prof["equal_edges"] == 2 * 19 * 19
