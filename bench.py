@@ -362,6 +362,14 @@ def main():
         "per_image_gpx": {name: round(npx / (m * 1e-3) / 1e9, 2) for (name, _), m in zip(images, med)},
         "last_stats": {k: v for k, v in st.items() if k != "status"},
     }
+    # seam sparsity of each image at the bench's tile size (the reference
+    # profiler's ratios), outside the timed region
+    sparsity = {}
+    for name, img in images:
+        row = ss.seam_profile_row(ss.seam_profile(img, tile, ws), side, side)
+        sparsity[name] = {k: round(row[k], 5) for k in ("seam_fragment_fraction", "seam_pixel_fraction",
+                                                        "seam_edge_fraction", "seam_fragments_per_mpix")}
+    out["seam_sparsity"] = sparsity
 
     # CPU-baseline sample (iid and sigma 4 / 16 levels when present), kept on the host
     pick = [i for i, (n_, _) in enumerate(images) if n_ in ("iid", "smooth_s4_q16")] or [0]

@@ -11,6 +11,17 @@
 //              [--levels=L] [--device=D] [--skip-expected]
 //
 // Exit codes: 0 all masks match, 1 mismatches, 2 usage, 3 error.
+//
+//   ss_harness --seam-profile <image_dir> --output=FILE [--tiles=16,32,64]
+//              [--levels=256,32,8,2] [--device=D]
+//
+// Seam-sparsity mode, mirroring the reference's profiler
+// (harness/seam_sparsity_profiler/gpu_seam_sparsity_profiler_v0_1.cu: CLI
+// :341-383, per image x levels x tiles rows :391-517, exit codes :518-540):
+// one CSV row per (image, levels, tile) with the same columns.  Tiles are
+// limited to the operator's 16, 32 and 64.  The reference quantises to bin
+// indices ((v L) >> 8) there; the rescaled quantiser used here maps the same
+// bins to distinct values, so every count is the same.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -108,9 +119,127 @@ std::string quote(const std::string& s) {
 
 bool starts(const std::string& s, const char* p) { return s.rfind(p, 0) == 0; }
 
+std::vector<int> int_list(const std::string& v) {
+    std::vector<int> out;
+    size_t b = 0;
+    while (b < v.size()) {
+        const size_t c = v.find(',', b);
+        out.push_back(std::stoi(v.substr(b, c == std::string::npos ? std::string::npos : c - b)));
+        if (c == std::string::npos) break;
+        b = c + 1;
+    }
+    return out;
+}
+
+std::vector<fs::path> pgm_files(const fs::path& dir) {
+    std::vector<fs::path> files;
+    for (const auto& e : fs::directory_iterator(dir)) {
+        std::string ext = e.path().extension().string();
+        std::transform(ext.begin(), ext.end(), ext.begin(), ::tolower);
+        if (e.is_regular_file() && ext == ".pgm") files.push_back(e.path());
+    }
+    std::sort(files.begin(), files.end());
+    return files;
+}
+
+int seam_profile_main(int argc, char** argv) {
+    if (argc < 3) {
+        std::cerr << "Usage: ss_harness --seam-profile <image_dir> --output=FILE [--tiles=16,32,64]"
+                     " [--levels=256,32,8,2] [--device=D]\n";
+        return 2;
+    }
+    try {
+        const fs::path image_dir = argv[2];
+        fs::path output;
+        std::vector<int> tiles{16, 32, 64}, levels{256, 32, 8, 2};
+        int device = 0;
+        for (int i = 3; i < argc; ++i) {
+            const std::string a = argv[i];
+            if (starts(a, "--output=")) output = a.substr(9);
+            else if (starts(a, "--tiles=")) tiles = int_list(a.substr(8));
+            else if (starts(a, "--levels=")) levels = int_list(a.substr(9));
+            else if (starts(a, "--device=")) device = std::stoi(a.substr(9));
+            else {
+                std::cerr << "Unknown argument " << a << '\n';
+                return 2;
+            }
+        }
+        if (output.empty()) throw std::runtime_error("Missing --output file");
+        for (int t : tiles)
+            if (t != 16 && t != 32 && t != 64) throw std::runtime_error("Tile size must be 16, 32 or 64");
+        for (int l : levels)
+            if (l < 2 || l > 256) throw std::runtime_error("Levels must be in [2,256]");
+        SS_HIP_CHECK(hipSetDevice(device));
+        if (output.has_parent_path()) fs::create_directories(output.parent_path());
+        std::ofstream csv(output);
+        if (!csv) throw std::runtime_error("Cannot create output CSV");
+        csv << "image,width,height,megapixels,levels,tile,"
+               "total_tile_fragments,seam_fragments,interior_fragments,"
+               "seam_fragment_fraction,seam_pixels,seam_pixel_fraction,"
+               "equal_forward_edges,equal_seam_edges,"
+               "seam_edge_fraction,seam_fragments_per_mpix,"
+               "equal_seam_edges_per_mpix,diagnostic_ms\n";
+        const std::vector<fs::path> files = pgm_files(image_dir);
+        for (const fs::path& f : files) {
+            const Pgm src = read_pgm(f);
+            const size_t px = src.data.size();
+            const double mp = (double)px / 1e6;
+            uint8_t* d_img = nullptr;
+            void* d_ws = nullptr;
+            size_t ws_bytes = 0;
+            for (int t : tiles) {
+                size_t need = 0;
+                const ss_config c{t, 256};
+                sparse_seam::check(ss_workspace_size(src.width, src.height, &c, &need), "ss_workspace_size");
+                ws_bytes = std::max(ws_bytes, need);
+            }
+            SS_HIP_CHECK(hipMalloc(&d_img, px));
+            SS_HIP_CHECK(hipMalloc(&d_ws, ws_bytes));
+            for (int level : levels) {
+                std::vector<uint8_t> q = src.data;
+                quantize(q, level);
+                SS_HIP_CHECK(hipMemcpy(d_img, q.data(), px, hipMemcpyHostToDevice));
+                for (int t : tiles) {
+                    const ss_config c{t, 256};
+                    ss_seam_profile p{};
+                    const auto t0 = std::chrono::high_resolution_clock::now();
+                    sparse_seam::check(ss_seam_profile_u8(d_img, (size_t)src.width, src.width, src.height, &c,
+                                                          d_ws, ws_bytes, nullptr, &p),
+                                       "ss_seam_profile_u8");
+                    const double ms =
+                        std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0)
+                            .count();
+                    const double sff = p.fragments ? (double)p.seam_fragments / (double)p.fragments : 0.0;
+                    const double spf = (double)p.seam_pixels / (double)px;
+                    const double sef = p.equal_edges ? (double)p.seam_edges / (double)p.equal_edges : 0.0;
+                    csv << quote(f.filename().string()) << ',' << src.width << ',' << src.height << ','
+                        << std::fixed << std::setprecision(6) << mp << ',' << level << ',' << t << ','
+                        << p.fragments << ',' << p.seam_fragments << ',' << p.fragments - p.seam_fragments << ','
+                        << sff << ',' << p.seam_pixels << ',' << spf << ',' << p.equal_edges << ','
+                        << p.seam_edges << ',' << sef << ',' << (double)p.seam_fragments / mp << ','
+                        << (double)p.seam_edges / mp << ',' << ms << '\n';
+                    csv.flush();
+                    std::cout << "[SS-SEAM-PROFILE] " << f.filename().string() << " q=" << level << " tile=" << t
+                              << " fragments=" << p.fragments << " seam_fragments=" << p.seam_fragments
+                              << " seam_fragment_fraction=" << sff << " seam_pixel_fraction=" << spf
+                              << " seam_edges=" << p.seam_edges << '\n';
+                }
+            }
+            SS_HIP_CHECK(hipFree(d_img));
+            SS_HIP_CHECK(hipFree(d_ws));
+        }
+        std::cout << "[SS-SEAM-PROFILE-OK] images=" << files.size() << " output=" << output.string() << '\n';
+        return 0;
+    } catch (const std::exception& e) {
+        std::cerr << "[ERROR] " << e.what() << '\n';
+        return 3;
+    }
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
+    if (argc >= 2 && std::string(argv[1]) == "--seam-profile") return seam_profile_main(argc, argv);
     if (argc < 3) {
         std::cerr << "Usage: ss_harness <image_dir> --output=DIR [--expected-max=DIR --expected-min=DIR]"
                      " [--warmup=5] [--reps=30] [--tile=T --threads=N | --autotune] [--levels=L]"
@@ -151,13 +280,7 @@ int main(int argc, char** argv) {
         hipDeviceProp_t prop{};
         SS_HIP_CHECK(hipGetDeviceProperties(&prop, device));
 
-        std::vector<fs::path> files;
-        for (const auto& e : fs::directory_iterator(image_dir)) {
-            std::string ext = e.path().extension().string();
-            std::transform(ext.begin(), ext.end(), ext.begin(), ::tolower);
-            if (e.is_regular_file() && ext == ".pgm") files.push_back(e.path());
-        }
-        std::sort(files.begin(), files.end());
+        const std::vector<fs::path> files = pgm_files(image_dir);
 
         std::vector<Pgm> images;
         for (const auto& f : files) {

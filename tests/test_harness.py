@@ -6,7 +6,7 @@ import subprocess
 
 import pytest
 
-from sscommon import GOLDEN, REPO, SUITES
+from sscommon import GOLDEN, REPO, SUITES, oracle_quantize, oracle_seam_profile, read_pgm
 
 HARNESS = os.path.join(REPO, "sparse-seam_amd", "ss_harness")
 
@@ -23,6 +23,12 @@ def test_usage_and_argument_errors():
     assert r.returncode == 3 and "Missing --output" in r.stderr
     r = run([GOLDEN, "--output=/tmp/x", "--levels=1"])
     assert r.returncode == 3
+    assert run(["--seam-profile"]).returncode == 2
+    assert run(["--seam-profile", GOLDEN, "--bogus"]).returncode == 2
+    r = run(["--seam-profile", GOLDEN])           # no --output
+    assert r.returncode == 3 and "Missing --output" in r.stderr
+    r = run(["--seam-profile", GOLDEN, "--output=/tmp/x.csv", "--tiles=128"])
+    assert r.returncode == 3 and "Tile size" in r.stderr
 
 
 @pytest.mark.gpu
@@ -57,3 +63,31 @@ def test_autotune_and_levels(tmp_path):
              "--autotune", "--levels=4", "--warmup=1", "--reps=2"])
     assert r.returncode == 0, r.stderr[-2000:]
     assert "[SS-AUTOTUNE]" in r.stdout and "[SS-HARNESS-MEASUREMENT-OK]" in r.stdout
+
+
+@pytest.mark.gpu
+def test_seam_profile_mode(tmp_path):
+    """Seam-sparsity rows with the reference profiler's columns; counts equal
+    the oracle restatement on the quantised inputs."""
+    root = os.path.join(GOLDEN, "oracle_v0_3_sparse_seam_adversarial", "inputs")
+    out = tmp_path / "profile.csv"
+    r = run(["--seam-profile", root, "--output=%s" % out, "--tiles=16,64", "--levels=256,4"])
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "[SS-SEAM-PROFILE-OK]" in r.stdout
+    rows = list(csv.DictReader(open(out)))
+    names = sorted(os.listdir(root))
+    assert len(rows) == len(names) * 4
+    assert list(rows[0].keys()) == [
+        "image", "width", "height", "megapixels", "levels", "tile", "total_tile_fragments",
+        "seam_fragments", "interior_fragments", "seam_fragment_fraction", "seam_pixels",
+        "seam_pixel_fraction", "equal_forward_edges", "equal_seam_edges", "seam_edge_fraction",
+        "seam_fragments_per_mpix", "equal_seam_edges_per_mpix", "diagnostic_ms"]
+    for row in rows[:8]:
+        img = oracle_quantize(read_pgm(os.path.join(root, row["image"])), int(row["levels"]))
+        want = oracle_seam_profile(img, int(row["tile"]))
+        assert int(row["total_tile_fragments"]) == want["fragments"]
+        assert int(row["seam_fragments"]) == want["seam_fragments"]
+        assert int(row["seam_pixels"]) == want["seam_pixels"]
+        assert int(row["equal_forward_edges"]) == want["equal_edges"]
+        assert int(row["equal_seam_edges"]) == want["seam_edges"]
+        assert int(row["interior_fragments"]) == want["fragments"] - want["seam_fragments"]
