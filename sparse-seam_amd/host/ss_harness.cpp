@@ -318,6 +318,14 @@ int main(int argc, char** argv) {
                "oracle_max_mismatches,oracle_min_mismatches,"
                "cross_max_mismatches,cross_min_mismatches,"
                "tile,active_fragments,dirty_tiles,dsu_retries\n";
+        // per-image validation rows with the columns of the reference's
+        // snapshot harness (snapshot_sparse_seam_cuda_v1_0.cu:931-933,1000-1014);
+        // auxiliary bytes are this build's workspace (the masks and the image
+        // are the caller's)
+        std::ofstream vcsv(out_dir / "validation.csv");
+        vcsv << "image,width,height,tiles,fragment_capacity,active_fragments,"
+                "auxiliary_bytes,auxiliary_bytes_per_pixel,elapsed_ms,"
+                "oracle_checked,max_mismatches,min_mismatches\n";
         {
             std::ofstream env(out_dir / "environment.txt");
             env << "harness=ss_harness\n"
@@ -393,6 +401,14 @@ int main(int argc, char** argv) {
                 << ",0,0," << cfg.tile << ',' << st.fragments << ',' << st.dirty_tiles << ',' << st.dsu_retries
                 << '\n';
             csv.flush();
+            const uint64_t tiles = (uint64_t)((img.width + cfg.tile - 1) / cfg.tile) *
+                                   (uint64_t)((img.height + cfg.tile - 1) / cfg.tile);
+            vcsv << quote(files[k].filename().string()) << ',' << img.width << ',' << img.height << ',' << tiles
+                 << ',' << tiles * (uint64_t)(4 * cfg.tile - 4) << ',' << st.fragments << ','
+                 << b.workspace_bytes << ',' << std::fixed << std::setprecision(6)
+                 << (double)b.workspace_bytes / px << ',' << med_c << ',' << (skip_expected ? 0 : 1) << ','
+                 << mm_max << ',' << mm_min << '\n';
+            vcsv.flush();
             std::cout << "[SS-HARNESS] " << files[k].filename().string() << ' ' << img.width << 'x'
                       << img.height << " compute_ms=" << med_c << " h2h_ms=" << med_h
                       << " fragments=" << st.fragments << " max_mismatch=" << mm_max

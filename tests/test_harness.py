@@ -1,14 +1,17 @@
 """The C++ command-line gate (sparse-seam_amd/ss_harness): exit codes, CSV,
 and the reference's oracle gate on the 206 golden triples."""
 import csv
+import json
 import os
 import subprocess
+import sys
 
 import pytest
 
 from sscommon import GOLDEN, REPO, SUITES, oracle_quantize, oracle_seam_profile, read_pgm
 
 HARNESS = os.path.join(REPO, "sparse-seam_amd", "ss_harness")
+AGGREGATE = os.path.join(REPO, "sparse-seam_amd", "host", "ss_aggregate.py")
 
 
 def run(args, timeout=300):
@@ -45,6 +48,13 @@ def test_golden_gate(tmp_path, suite):
     assert len(rows) == len(os.listdir(os.path.join(root, "inputs")))
     assert all(row["oracle_max_mismatches"] == "0" and row["oracle_min_mismatches"] == "0" for row in rows)
     assert (out / "environment.txt").exists()
+    val = list(csv.DictReader(open(out / "validation.csv")))
+    assert list(val[0].keys()) == ["image", "width", "height", "tiles", "fragment_capacity", "active_fragments",
+                                   "auxiliary_bytes", "auxiliary_bytes_per_pixel", "elapsed_ms", "oracle_checked",
+                                   "max_mismatches", "min_mismatches"]
+    assert len(val) == len(rows) and all(v["oracle_checked"] == "1" and v["max_mismatches"] == "0" for v in val)
+    agg = subprocess.run([sys.executable, AGGREGATE, str(out / "ss_harness.csv")], capture_output=True, text=True)
+    assert agg.returncode == 0, agg.stdout + agg.stderr
 
 
 @pytest.mark.gpu
@@ -91,3 +101,25 @@ def test_seam_profile_mode(tmp_path):
         assert int(row["equal_forward_edges"]) == want["equal_edges"]
         assert int(row["equal_seam_edges"]) == want["seam_edges"]
         assert int(row["interior_fragments"]) == want["fragments"] - want["seam_fragments"]
+
+
+def test_aggregate_sum_px_over_sum_median(tmp_path):
+    """Throughput = total pixels / sum of per-image median times (the
+    reference aggregator's rule), mismatches summed, exit 1 on a mismatch."""
+    cols = ["algorithm", "image", "width", "height", "median_compute_ms", "median_h2h_ms", "auxiliary_bytes",
+            "oracle_max_mismatches", "oracle_min_mismatches", "cross_max_mismatches", "cross_min_mismatches",
+            "tile"]
+    path = tmp_path / "ss_harness.csv"
+    with open(path, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(cols)
+        w.writerow(["A", "a.pgm", 1000, 1000, 2.0, 8.0, 500000, 0, 0, 0, 0, 64])
+        w.writerow(["A", "b.pgm", 2000, 1000, 6.0, 12.0, 1000000, 0, 0, 0, 0, 64])
+    r = subprocess.run([sys.executable, AGGREGATE, str(path)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    a = json.loads(r.stdout.strip().splitlines()[-1])["A/T64"]
+    assert a["pixels"] == 3_000_000 and abs(a["compute_mpix_s"] - 3.0 / 0.008) < 1e-9
+    assert abs(a["h2h_mpix_s"] - 3.0 / 0.020) < 1e-9 and abs(a["aux_Bpp"] - 0.5) < 1e-12
+    with open(path, "a", newline="") as f:
+        csv.writer(f).writerow(["A", "c.pgm", 10, 10, 1.0, 1.0, 100, 1, 0, 0, 0, 64])
+    assert subprocess.run([sys.executable, AGGREGATE, str(path)], capture_output=True).returncode == 1
