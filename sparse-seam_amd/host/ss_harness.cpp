@@ -4,13 +4,19 @@
 // CLI :500-535, per-image loop :584-745, timing protocol :305-397, CSV columns
 // :545-553, percentile rule :86-103, byte mask compare :105-121, exit codes
 // :752-755 and :1068-1076 of snapshot_sparse_seam_cuda_v1_0.cu) on top of the
-// C ABI.  Images are binary PGM (P5); OpenCV is not used.
+// C ABI.  Images are binary PGM (P5) or PNG (8-bit gray, gray + alpha,
+// palette, RGB or RGBA; colour converted to gray with the ITU-R 601 weights
+// in 15-bit fixed point, alpha dropped); OpenCV is not used.
 //
 //   ss_harness <image_dir> --output=DIR [--expected-max=DIR --expected-min=DIR]
 //              [--warmup=5] [--reps=30] [--tile=T --threads=N | --autotune]
 //              [--levels=L] [--device=D] [--skip-expected]
 //
 // Exit codes: 0 all masks match, 1 mismatches, 2 usage, 3 error.
+//
+//   ss_harness --convert <image_dir> --output=DIR
+//
+// Writes every input as the 8-bit gray PGM the other modes read (no GPU).
 //
 //   ss_harness --seam-profile <image_dir> --output=FILE [--tiles=16,32,64]
 //              [--levels=256,32,8,2] [--device=D]
@@ -35,6 +41,8 @@
 #include <iostream>
 #include <string>
 #include <vector>
+
+#include <zlib.h>
 
 #include "sparse_seam.hpp"
 
@@ -75,6 +83,108 @@ Pgm read_pgm(const fs::path& path) {
     if (buf.size() < pos + n) throw std::runtime_error("Truncated PGM data " + path.string());
     p.data.assign(buf.begin() + pos, buf.begin() + pos + n);
     return p;
+}
+
+uint32_t be32(const uint8_t* p) { return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3]; }
+
+// 8-bit gray from R, G, B: 0.299, 0.587, 0.114 in 15-bit fixed point.
+uint8_t rgb_gray(uint32_t r, uint32_t g, uint32_t b) {
+    return (uint8_t)((r * 9798u + g * 19235u + b * 3735u + 16384u) >> 15);
+}
+
+// PNG reader: chunks (CRC checked), zlib inflate of the IDAT stream, the five
+// row filters, non-interlaced 8-bit images of colour types 0, 2, 3, 4, 6.
+Pgm read_png(const fs::path& path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("Cannot read input " + path.string());
+    const std::vector<uint8_t> buf((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', 0x0D, 0x0A, 0x1A, 0x0A};
+    if (buf.size() < 8 || std::memcmp(buf.data(), sig, 8) != 0) throw std::runtime_error("Not a PNG " + path.string());
+    int w = 0, h = 0, depth = 0, ctype = -1;
+    std::vector<uint8_t> idat, plte;
+    size_t pos = 8;
+    bool end = false;
+    while (!end) {
+        if (pos + 12 > buf.size()) throw std::runtime_error("Truncated PNG " + path.string());
+        const uint32_t len = be32(&buf[pos]);
+        if (len > buf.size() - pos - 12) throw std::runtime_error("Truncated PNG chunk " + path.string());
+        const uint8_t* type = &buf[pos + 4];
+        const uint8_t* data = &buf[pos + 8];
+        if ((uint32_t)crc32(crc32(0L, Z_NULL, 0), type, len + 4) != be32(data + len))
+            throw std::runtime_error("PNG chunk CRC mismatch " + path.string());
+        if (!std::memcmp(type, "IHDR", 4)) {
+            if (len != 13) throw std::runtime_error("Bad PNG header " + path.string());
+            w = (int)be32(data);
+            h = (int)be32(data + 4);
+            depth = data[8];
+            ctype = data[9];
+            if (data[10] != 0 || data[11] != 0 || data[12] != 0)
+                throw std::runtime_error("Unsupported PNG (interlaced or non-standard) " + path.string());
+        } else if (!std::memcmp(type, "PLTE", 4)) {
+            plte.assign(data, data + len);
+        } else if (!std::memcmp(type, "IDAT", 4)) {
+            idat.insert(idat.end(), data, data + len);
+        } else if (!std::memcmp(type, "IEND", 4)) {
+            end = true;
+        }
+        pos += 12 + len;
+    }
+    const int ch = ctype == 0 ? 1 : ctype == 2 ? 3 : ctype == 3 ? 1 : ctype == 4 ? 2 : ctype == 6 ? 4 : 0;
+    if (w <= 0 || h <= 0 || depth != 8 || ch == 0 || (ctype == 3 && plte.size() < 3))
+        throw std::runtime_error("Unsupported PNG (8-bit gray, palette, RGB or alpha only) " + path.string());
+    const size_t stride = (size_t)w * ch;
+    std::vector<uint8_t> raw((stride + 1) * (size_t)h);
+    uLongf rlen = (uLongf)raw.size();
+    if (uncompress(raw.data(), &rlen, idat.data(), (uLong)idat.size()) != Z_OK || rlen != raw.size())
+        throw std::runtime_error("Corrupt PNG data " + path.string());
+    std::vector<uint8_t> img(stride * (size_t)h);
+    for (int y = 0; y < h; ++y) {
+        const uint8_t f = raw[(stride + 1) * y];
+        const uint8_t* src = &raw[(stride + 1) * y + 1];
+        uint8_t* cur = &img[stride * y];
+        const uint8_t* prev = y ? &img[stride * (y - 1)] : nullptr;
+        for (size_t i = 0; i < stride; ++i) {
+            const int a = i >= (size_t)ch ? cur[i - ch] : 0, b = prev ? prev[i] : 0;
+            const int c = (prev && i >= (size_t)ch) ? prev[i - ch] : 0;
+            int pred = 0;
+            switch (f) {
+            case 0: pred = 0; break;
+            case 1: pred = a; break;
+            case 2: pred = b; break;
+            case 3: pred = (a + b) >> 1; break;
+            case 4: {
+                const int p = a + b - c, pa = std::abs(p - a), pb = std::abs(p - b), pc = std::abs(p - c);
+                pred = (pa <= pb && pa <= pc) ? a : (pb <= pc ? b : c);
+                break;
+            }
+            default: throw std::runtime_error("Bad PNG filter " + path.string());
+            }
+            cur[i] = (uint8_t)(src[i] + pred);
+        }
+    }
+    Pgm p;
+    p.width = w;
+    p.height = h;
+    p.data.resize((size_t)w * h);
+    for (size_t i = 0; i < p.data.size(); ++i) {
+        const uint8_t* px = &img[i * ch];
+        if (ctype == 0 || ctype == 4) {
+            p.data[i] = px[0];
+        } else if (ctype == 3) {
+            if ((size_t)px[0] * 3 + 2 >= plte.size()) throw std::runtime_error("PNG palette index " + path.string());
+            const uint8_t* e = &plte[(size_t)px[0] * 3];
+            p.data[i] = rgb_gray(e[0], e[1], e[2]);
+        } else {
+            p.data[i] = rgb_gray(px[0], px[1], px[2]);
+        }
+    }
+    return p;
+}
+
+Pgm read_image(const fs::path& path) {
+    std::string ext = path.extension().string();
+    std::transform(ext.begin(), ext.end(), ext.begin(), ::tolower);
+    return ext == ".png" ? read_png(path) : read_pgm(path);
 }
 
 void write_pgm(const fs::path& path, int w, int h, const uint8_t* data) {
@@ -131,15 +241,36 @@ std::vector<int> int_list(const std::string& v) {
     return out;
 }
 
-std::vector<fs::path> pgm_files(const fs::path& dir) {
+std::vector<fs::path> image_files(const fs::path& dir) {
     std::vector<fs::path> files;
     for (const auto& e : fs::directory_iterator(dir)) {
         std::string ext = e.path().extension().string();
         std::transform(ext.begin(), ext.end(), ext.begin(), ::tolower);
-        if (e.is_regular_file() && ext == ".pgm") files.push_back(e.path());
+        if (e.is_regular_file() && (ext == ".pgm" || ext == ".png")) files.push_back(e.path());
     }
     std::sort(files.begin(), files.end());
     return files;
+}
+
+int convert_main(int argc, char** argv) {
+    if (argc < 4 || !starts(argv[3], "--output=")) {
+        std::cerr << "Usage: ss_harness --convert <image_dir> --output=DIR\n";
+        return 2;
+    }
+    try {
+        const fs::path out = std::string(argv[3]).substr(9);
+        fs::create_directories(out);
+        const std::vector<fs::path> files = image_files(argv[2]);
+        for (const fs::path& f : files) {
+            const Pgm p = read_image(f);
+            write_pgm(out / (f.stem().string() + ".pgm"), p.width, p.height, p.data.data());
+        }
+        std::cout << "[SS-CONVERT-OK] images=" << files.size() << " output=" << out.string() << '\n';
+        return 0;
+    } catch (const std::exception& e) {
+        std::cerr << "[ERROR] " << e.what() << '\n';
+        return 3;
+    }
 }
 
 int seam_profile_main(int argc, char** argv) {
@@ -179,9 +310,9 @@ int seam_profile_main(int argc, char** argv) {
                "equal_forward_edges,equal_seam_edges,"
                "seam_edge_fraction,seam_fragments_per_mpix,"
                "equal_seam_edges_per_mpix,diagnostic_ms\n";
-        const std::vector<fs::path> files = pgm_files(image_dir);
+        const std::vector<fs::path> files = image_files(image_dir);
         for (const fs::path& f : files) {
-            const Pgm src = read_pgm(f);
+            const Pgm src = read_image(f);
             const size_t px = src.data.size();
             const double mp = (double)px / 1e6;
             uint8_t* d_img = nullptr;
@@ -240,6 +371,7 @@ int seam_profile_main(int argc, char** argv) {
 
 int main(int argc, char** argv) {
     if (argc >= 2 && std::string(argv[1]) == "--seam-profile") return seam_profile_main(argc, argv);
+    if (argc >= 2 && std::string(argv[1]) == "--convert") return convert_main(argc, argv);
     if (argc < 3) {
         std::cerr << "Usage: ss_harness <image_dir> --output=DIR [--expected-max=DIR --expected-min=DIR]"
                      " [--warmup=5] [--reps=30] [--tile=T --threads=N | --autotune] [--levels=L]"
@@ -280,11 +412,11 @@ int main(int argc, char** argv) {
         hipDeviceProp_t prop{};
         SS_HIP_CHECK(hipGetDeviceProperties(&prop, device));
 
-        const std::vector<fs::path> files = pgm_files(image_dir);
+        const std::vector<fs::path> files = image_files(image_dir);
 
         std::vector<Pgm> images;
         for (const auto& f : files) {
-            images.push_back(read_pgm(f));
+            images.push_back(read_image(f));
             quantize(images.back().data, levels);
         }
 

@@ -123,3 +123,62 @@ def test_aggregate_sum_px_over_sum_median(tmp_path):
     with open(path, "a", newline="") as f:
         csv.writer(f).writerow(["A", "c.pgm", 10, 10, 1.0, 1.0, 100, 1, 0, 0, 0, 64])
     assert subprocess.run([sys.executable, AGGREGATE, str(path)], capture_output=True).returncode == 1
+
+
+def test_png_reader(tmp_path):
+    """PNG decoding of the harness (--convert, no GPU): the reference's own PNG
+    fixtures equal its PGM inputs; PIL-written gray, gray + alpha, palette, RGB
+    and RGBA images decode to the ITU-R 601 gray of their pixels; 16-bit and
+    interlaced files are refused."""
+    Image = pytest.importorskip("PIL.Image")
+    import numpy as np
+    src = os.path.join(GOLDEN, "oracle_v0_1", "inputs_png")
+    out = tmp_path / "conv"
+    r = run(["--convert", src, "--output=%s" % out])
+    assert r.returncode == 0, r.stderr
+    for name in os.listdir(src):
+        stem = name[:-4]
+        assert np.array_equal(read_pgm(str(out / (stem + ".pgm"))),
+                              read_pgm(os.path.join(GOLDEN, "oracle_v0_1", "inputs", stem + ".pgm"))), name
+    rng = np.random.default_rng(3)
+    d = tmp_path / "png"
+    d.mkdir()
+    rgb = rng.integers(0, 256, (37, 53, 3), dtype=np.uint8)
+    rgb[5:20, 10:30] = 200                       # flat areas exercise every filter type
+    gray601 = ((rgb[..., 0].astype(np.uint32) * 9798 + rgb[..., 1].astype(np.uint32) * 19235 +
+                rgb[..., 2].astype(np.uint32) * 3735 + 16384) >> 15).astype(np.uint8)
+    g = rng.integers(0, 256, (41, 29), dtype=np.uint8)
+    want = {}
+    Image.fromarray(g, "L").save(d / "gray.png")
+    want["gray"] = g
+    Image.fromarray(np.dstack([g, 255 - g]), "LA").save(d / "gray_alpha.png")
+    want["gray_alpha"] = g
+    Image.fromarray(rgb, "RGB").save(d / "rgb.png")
+    want["rgb"] = gray601
+    Image.fromarray(np.dstack([rgb, rng.integers(0, 256, rgb.shape[:2], dtype=np.uint8)]), "RGBA").save(d / "rgba.png")
+    want["rgba"] = gray601
+    pal = Image.fromarray(rgb, "RGB").quantize(colors=17)
+    pal.save(d / "palette.png")
+    prgb = np.array(pal.convert("RGB")).astype(np.uint32)
+    want["palette"] = ((prgb[..., 0] * 9798 + prgb[..., 1] * 19235 + prgb[..., 2] * 3735 + 16384) >> 15).astype(np.uint8)
+    out2 = tmp_path / "conv2"
+    r = run(["--convert", str(d), "--output=%s" % out2])
+    assert r.returncode == 0, r.stderr
+    for name, arr in want.items():
+        assert np.array_equal(read_pgm(str(out2 / (name + ".pgm"))), arr), name
+    bad = tmp_path / "bad"
+    bad.mkdir()
+    Image.fromarray(g.astype(np.uint16) * 257).save(bad / "deep.png")
+    r = run(["--convert", str(bad), "--output=%s" % (tmp_path / "conv3")])
+    assert r.returncode == 3 and "Unsupported PNG" in r.stderr
+
+
+@pytest.mark.gpu
+def test_golden_gate_png_inputs(tmp_path):
+    """The gate on the reference's PNG copies of its first golden suite."""
+    root = os.path.join(GOLDEN, "oracle_v0_1")
+    r = run([os.path.join(root, "inputs_png"), "--output=%s" % (tmp_path / "o"),
+             "--expected-max=%s" % os.path.join(root, "expected_max"),
+             "--expected-min=%s" % os.path.join(root, "expected_min"), "--warmup=0", "--reps=1"])
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "[SS-HARNESS-VALIDATION-OK] images=34 mismatches=0" in r.stdout
