@@ -237,6 +237,37 @@ __device__ __forceinline__ void tile_load(TileSmem<T>& sm, const ImgDev& im, int
     constexpr int ITEMS = ((T + 2) * D + NT - 1) / NT;   // dwords per thread
     constexpr int HALO = (2 * (T + 2) + NT - 1) / NT;    // halo-column bytes per thread
     const bool al4 = ((reinterpret_cast<uintptr_t>(im.img) | (uintptr_t)im.pitch) & 3u) == 0;
+    if (al4 && x0 > 0 && y0 > 0 && x0 + T < im.w && y0 + T < im.h) {
+        // interior tile: every staged row and column lies in the image, so
+        // the addresses are one uniform base plus per-thread offsets
+        const uint8_t* base = im.img + (int64_t)(y0 - 1) * im.pitch + x0;
+        uint32_t v[ITEMS];
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int it = threadIdx.x + i * NT;
+            v[i] = 0;
+            if (it < (T + 2) * D)
+                v[i] = *reinterpret_cast<const uint32_t*>(base + (int64_t)(it / D) * im.pitch + 4 * (it % D));
+        }
+        uint8_t hv[HALO];
+#pragma unroll
+        for (int i = 0; i < HALO; ++i) {
+            const int it = threadIdx.x + i * NT;
+            hv[i] = 0;
+            if (it < 2 * (T + 2)) hv[i] = base[(int64_t)(it / 2) * im.pitch + ((it & 1) ? T : -1)];
+        }
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int it = threadIdx.x + i * NT;
+            if (it < (T + 2) * D) *reinterpret_cast<uint32_t*>(&sm.img[(it / D) * P + 4 + 4 * (it % D)]) = v[i];
+        }
+#pragma unroll
+        for (int i = 0; i < HALO; ++i) {
+            const int it = threadIdx.x + i * NT;
+            if (it < 2 * (T + 2)) sm.img[(it / 2) * P + 4 + ((it & 1) ? T : -1)] = hv[i];
+        }
+        return;
+    }
     auto row_of = [&](int r) -> const uint8_t* {
         int gy = y0 + r;
         gy = gy < 0 ? 0 : (gy >= im.h ? im.h - 1 : gy);
