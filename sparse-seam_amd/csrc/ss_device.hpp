@@ -31,6 +31,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ss_seam.hpp"
+
 namespace ssx {
 
 constexpr uint32_t F_G = 1u;      // a strictly greater 8-neighbour exists
@@ -44,7 +46,6 @@ constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_CHUNK = SCAN_BLOCK * SCAN_ITEMS;  // tiles per scan block
 constexpr int NUM_XCD = 8;
-constexpr int MERGE_BT = 8;        // tiles per side of a block-local seam merge
 constexpr int MERGE_THREADS = 1024;
 
 // One image of a launch set.  tile_begin is the global id of its first tile,

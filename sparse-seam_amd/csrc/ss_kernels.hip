@@ -380,12 +380,25 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
 // corner is both the up-right contact of one tile's right edge and the
 // down-left contact of the bottom edge of the tile above-right of it (resp.
 // the down-right contact of both edges of one tile).  Pairs inside one block
-// were united by k_block_merge.  A plateau crossing a seam yields the same
-// fragment pair at many consecutive k, so a pair already present in the
-// thread or in the neighbouring lane (the previous k of the same edge) is not
-// united again.
+// were united by k_block_merge.
+//
+// A plateau crossing a block seam yields the same pair of block components at
+// many pixels of the seam, and every union ends in a device-scope CAS (or
+// several, when they collide), which is what this kernel's time goes to.  So
+// the units of one block seam are consecutive (a workgroup of SEAM_THREADS
+// covers the MERGE_BT edges of one seam at T = 64), and each distinct pair of
+// block roots is united once per workgroup: the wave first keeps one lane per
+// distinct pair, then an LDS hash set drops the pairs another wave of the
+// workgroup already holds.
+constexpr int SEAM_THREADS = 512;
+constexpr int SEAM_SET = 2048;   // LDS hash-set slots (pairs a workgroup may keep: 3 per thread)
+
 template <int T, bool MULTI>
-__global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
+__global__ __launch_bounds__(SEAM_THREADS) void k_seam_merge(Images ims, Ws ws) {
+    constexpr unsigned long long EMPTY = ~0ull;
+    __shared__ unsigned long long set[SEAM_SET];
+    for (int i = threadIdx.x; i < SEAM_SET; i += SEAM_THREADS) set[i] = EMPTY;
+    __syncthreads();
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t unit = g / T;
     const int k = (int)(g % T);
@@ -397,20 +410,9 @@ __global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
     if (unit < ws.nseam) {
         const ImgDev im = MULTI ? ims.table[ws.seam_img[unit]] : ims.one;
         const uint32_t u = (uint32_t)(unit - im.seam_begin);
-        const uint32_t nv = (uint32_t)im.seam_cols * (uint32_t)im.tiles_y;
-        const bool right = u < nv;
-        // consecutive units lie on different block seams: neighbouring edges
-        // of one seam mostly join the same plateau, and taking them at the
-        // same time would only make their CAS collide
         int tx, ty;
-        if (right) {
-            tx = MERGE_BT * (int)(u % (uint32_t)im.seam_cols) + MERGE_BT - 1;
-            ty = (int)(u / (uint32_t)im.seam_cols);
-        } else {
-            const uint32_t v = u - nv;
-            ty = MERGE_BT * (int)(v % (uint32_t)im.seam_rows) + MERGE_BT - 1;
-            tx = (int)(v / (uint32_t)im.seam_rows);
-        }
+        const bool right = seam_unit_edge(u, (uint32_t)im.seam_cols, (uint32_t)im.seam_rows,
+                                          (uint32_t)im.tiles_x, (uint32_t)im.tiles_y, tx, ty);
         // by construction the tile right of (below) the edge exists
         const int64_t gtile = im.tile_begin + (int64_t)ty * im.tiles_x + tx;
         const uint32_t cnt = ws.counts[gtile] & COUNT_MASK;
@@ -441,19 +443,54 @@ __global__ __launch_bounds__(256) void k_seam_merge(Images ims, Ws ws) {
             }
         }
     }
-    // pairs of the previous k (lane - 1) on the same edge
-    const uint32_t pfa = __shfl_up(fa, 1, 64);
-    uint32_t pfb[3];
+    // Unions act on block roots: after the block merge a fragment's parent is
+    // its block root (or, once this kernel has linked that root, an ancestor
+    // of it), so the contacts of an edge collapse to the few pairs of block
+    // components that meet across it.  The wave keeps each distinct pair once
+    // (one leader per pair and round).
+    uint32_t ra = NONE, rb[3] = {NONE, NONE, NONE};
+    if (fa != NONE) {
+        ra = ws.dparent[fa];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) pfb[d] = __shfl_up(fb[d], 1, 64);
-    const bool has_prev = lane > 0 && k > 0;
+        for (int d = 0; d < 3; ++d)
+            if (fb[d] != NONE) rb[d] = ws.dparent[fb[d]];
+    }
+    bool live[3], keep[3] = {false, false, false};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        live[d] = rb[d] != NONE && !(d > 0 && rb[d] == rb[d - 1]) && !(d > 1 && rb[d] == rb[d - 2]);
+    for (;;) {
+        const uint64_t m0 = __ballot(live[0]), m1 = __ballot(live[1]), m2 = __ballot(live[2]);
+        if (!(m0 | m1 | m2)) break;
+        const int dl = m0 ? 0 : (m1 ? 1 : 2);
+        const int leader = __ffsll((unsigned long long)(m0 ? m0 : (m1 ? m1 : m2))) - 1;
+        const uint32_t la = __shfl(ra, leader, 64);
+        const uint32_t lb = __shfl(dl == 0 ? rb[0] : (dl == 1 ? rb[1] : rb[2]), leader, 64);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (d == dl && lane == leader) keep[d] = true;
+            if (live[d] && ra == la && rb[d] == lb) live[d] = false;   // clears the leader's pair too
+        }
+    }
+    // Workgroup set: a pair found there was kept by another wave.  A pair
+    // that finds no slot within 8 probes is simply united (a repeated union
+    // is harmless).
+    static_assert(SEAM_SET == 2048, "the hash below yields 11 bits");
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        if (fb[d] == NONE) continue;
-        bool dup = (d > 0 && fb[d] == fb[d - 1]) || (d > 1 && fb[d] == fb[d - 2]);
-        if (has_prev && pfa == fa) dup = dup || fb[d] == pfb[0] || fb[d] == pfb[1] || fb[d] == pfb[2];
-        if (!dup) gunion(ws.dparent, fa, fb[d], retries);
+        if (!keep[d]) continue;
+        const uint32_t lo = min(ra, rb[d]), hi = max(ra, rb[d]);
+        const unsigned long long key = (unsigned long long)lo << 32 | hi;
+        const uint32_t h = (lo * 0x9E3779B1u ^ hi * 0x85EBCA77u) >> 21;
+        for (int p = 0; p < 8; ++p) {
+            const unsigned long long old = atomicCAS(&set[(h + p) & (SEAM_SET - 1)], EMPTY, key);
+            if (old == EMPTY) break;
+            if (old == key) { keep[d] = false; break; }
+        }
     }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (keep[d]) gunion(ws.dparent, ra, rb[d], retries);
     const uint32_t r = wave_sum(retries);
     if (lane == 0 && r) atomicAdd((unsigned long long*)&ws.hdr->retries, (unsigned long long)r);
     // a perimeter record out of range would mean a broken tile stage: report
@@ -578,8 +615,8 @@ static void launch_merge(const Images& ims, const Ws& ws, hipStream_t s) {
     k_block_merge<T, MULTI><<<(unsigned)ws.nblk, MERGE_THREADS, 0, s>>>(ims, ws);
     const int64_t merge_threads = ws.nseam * T;
     if (merge_threads > 0) {
-        const unsigned merge_grid = (unsigned)((merge_threads + 255) / 256);
-        k_seam_merge<T, MULTI><<<merge_grid, 256, 0, s>>>(ims, ws);
+        const unsigned merge_grid = (unsigned)((merge_threads + SEAM_THREADS - 1) / SEAM_THREADS);
+        k_seam_merge<T, MULTI><<<merge_grid, SEAM_THREADS, 0, s>>>(ims, ws);
     }
 }
 

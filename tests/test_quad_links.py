@@ -21,3 +21,17 @@ def test_quad_links_exhaustive(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "1048576 cases, 0 mismatches" in r.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_seam_unit_order(tmp_path):
+    """Host check of the seam merge's unit order (ss_seam.hpp): a bijection onto
+    the block-boundary tile edges, one block seam per aligned run of units."""
+    exe = tmp_path / "seam_units_check"
+    src = os.path.join(REPO, "tests", "seam_units_check.cpp")
+    inc = os.path.join(REPO, "sparse-seam_amd", "csrc")
+    subprocess.run([HIPCC, "-x", "hip", "--offload-host-only", "-O2", "-std=c++17", "-I", inc, "-o", str(exe), src],
+                   check=True, capture_output=True, text=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "4096 grids, 0 failed" in r.stdout
