@@ -170,24 +170,23 @@ __device__ __forceinline__ uint32_t lfind(const uint32_t* lab, uint32_t x) {
     }
 }
 
-// LDS find with path halving (plain labels, no flag bits).
-__device__ __forceinline__ uint32_t lfind_halve(uint32_t* lab, uint32_t x) {
+// Min-root LDS union (plain labels, no flag bits).  The two walks advance
+// together, one path-halving step each per round, so a union waits for the
+// deeper walk rather than for both in turn; a failed CAS resumes from the
+// parent it returns.
+__device__ __forceinline__ void lunion_pair(uint32_t* lab, uint32_t a, uint32_t b) {
     for (;;) {
-        const uint32_t p = lds_load(lab + x);
-        if (p == x) return x;
-        const uint32_t gp = lds_load(lab + p);
-        if (gp == p) return p;
-        __hip_atomic_store(lab + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        x = gp;
-    }
-}
-
-// Min-root LDS union; a failed CAS resumes from the parent it returns.
-__device__ __forceinline__ void lunion_halve(uint32_t* lab, uint32_t a, uint32_t b) {
-    for (;;) {
-        a = lfind_halve(lab, a);
-        b = lfind_halve(lab, b);
-        if (a == b) return;
+        for (;;) {
+            if (a == b) return;
+            const uint32_t pa = lds_load(lab + a), pb = lds_load(lab + b);
+            if (pa == a && pb == b) break;
+            // at a root, lab[pa] is pa itself and nothing is stored
+            const uint32_t ga = lds_load(lab + pa), gb = lds_load(lab + pb);
+            if (ga != pa) __hip_atomic_store(lab + a, ga, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (gb != pb) __hip_atomic_store(lab + b, gb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            a = ga;
+            b = gb;
+        }
         const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
         uint32_t expected = hi;
         if (__hip_atomic_compare_exchange_strong(lab + hi, &expected, lo, __ATOMIC_RELAXED,

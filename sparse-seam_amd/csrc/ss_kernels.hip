@@ -334,7 +334,7 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
             if (has_prev) dup = dup || bs[d] == pbs[0] || bs[d] == pbs[1] || bs[d] == pbs[2];
             if (dup) continue;
             if (in_lds) {
-                lunion_halve(par, a, bs[d]);
+                lunion_pair(par, a, bs[d]);
             } else {
                 gunion(ws.dparent, gb[ta] + (a - lb[ta]), gb[tb[d]] + (bs[d] - lb[tb[d]]), retries);
             }
