@@ -17,7 +17,9 @@
 //                   becomes its block-local root (dense arrays initialised);
 //                   b) lock-free min-root CAS union with path halving over the
 //                   dense ids for the seams between blocks, one thread per
-//                   pixel of a tile edge on a block boundary.
+//                   pixel of a tile edge on a block boundary; the edges of a
+//                   block seam are consecutive and a workgroup unites each
+//                   distinct pair of block roots once.
 //   4. resolve      full compression and OR of the local evidence into roots.
 //   5. mark and     one wave per tile holding a fragment that may still change:
 //      write-back   fragments whose resolved evidence differs from the local
