@@ -154,6 +154,22 @@ def test_large_images_vs_global_uf(dev):
             assert np.array_equal(mx, emax) and np.array_equal(mn, emin), name
 
 
+def test_full_size_sweep_vs_global_uf(dev):
+    """The benchmark's workload at full size and its default configuration
+    (T = 64, 256 threads): every image of the 16384x16384 sweep, bit-exact
+    against the CPU Global-UF route (one image is a second or two of CPU
+    work on the box's cores)."""
+    ws = None
+    for name, img in ssgen.sweep_images(16384, 0, device="cuda"):
+        say("full-size sweep 16384 %s" % name)
+        mx, mn, ws = ss.extrema(img, 64, 256, ws)
+        torch.cuda.synchronize()
+        assert ss.stats(ws)["invariant"] == 0
+        emax, emin = oracle_global_uf(img.cpu().numpy())
+        assert np.array_equal(mx.cpu().numpy(), emax), name
+        assert np.array_equal(mn.cpu().numpy(), emin), name
+
+
 def test_strided_rows_and_outputs(dev):
     rng = np.random.default_rng(9)
     base = torch.as_tensor(rng.integers(0, 4, (130, 300), dtype=np.uint8)).cuda()
