@@ -224,6 +224,17 @@ __device__ __forceinline__ void lunion_h(uint32_t* lab, uint32_t a, uint32_t b) 
     }
 }
 
+// Image and mask rows may start at any byte (pitches need not be multiples of
+// 4): gfx950 performs unaligned global loads and stores, and a memcpy of 4 or
+// 16 bytes compiles to one dword / dwordx4 access whatever the address.
+__device__ __forceinline__ uint32_t load_u32(const uint8_t* p) {
+    uint32_t v;
+    __builtin_memcpy(&v, p, 4);
+    return v;
+}
+__device__ __forceinline__ void store_u32(uint8_t* p, uint32_t v) { __builtin_memcpy(p, &v, 4); }
+__device__ __forceinline__ void store_u128(uint8_t* p, const uint4& v) { __builtin_memcpy(p, &v, 16); }
+
 // Loads the tile and its halo rows -1, th and halo columns -1, tw.  Halo
 // pixels outside the image replicate the nearest image pixel: such a value
 // belongs to the pixel itself or to one of its genuine 8-neighbours, so the
@@ -236,8 +247,7 @@ __device__ __forceinline__ void tile_load(TileSmem<T>& sm, const ImgDev& im, int
     constexpr int P = TileSmem<T>::P, D = T / 4;
     constexpr int ITEMS = ((T + 2) * D + NT - 1) / NT;   // dwords per thread
     constexpr int HALO = (2 * (T + 2) + NT - 1) / NT;    // halo-column bytes per thread
-    const bool al4 = ((reinterpret_cast<uintptr_t>(im.img) | (uintptr_t)im.pitch) & 3u) == 0;
-    if (al4 && x0 > 0 && y0 > 0 && x0 + T < im.w && y0 + T < im.h) {
+    if (x0 > 0 && y0 > 0 && x0 + T < im.w && y0 + T < im.h) {
         // interior tile: every staged row and column lies in the image, so
         // the addresses are one uniform base plus per-thread offsets
         const uint8_t* base = im.img + (int64_t)(y0 - 1) * im.pitch + x0;
@@ -247,7 +257,7 @@ __device__ __forceinline__ void tile_load(TileSmem<T>& sm, const ImgDev& im, int
             const int it = threadIdx.x + i * NT;
             v[i] = 0;
             if (it < (T + 2) * D)
-                v[i] = *reinterpret_cast<const uint32_t*>(base + (int64_t)(it / D) * im.pitch + 4 * (it % D));
+                v[i] = load_u32(base + (int64_t)(it / D) * im.pitch + 4 * (it % D));
         }
         uint8_t hv[HALO];
 #pragma unroll
@@ -281,8 +291,8 @@ __device__ __forceinline__ void tile_load(TileSmem<T>& sm, const ImgDev& im, int
         v[i] = 0;
         if (it >= (th + 2) * D || c >= tw) continue;
         const uint8_t* src = row_of(r) + c;
-        if (al4 && c + 3 < tw) {
-            v[i] = *reinterpret_cast<const uint32_t*>(src);
+        if (c + 3 < tw) {
+            v[i] = load_u32(src);
         } else {
             for (int b = 0; b < 4 && c + b < tw; ++b) v[i] |= (uint32_t)src[b] << (8 * b);
         }
@@ -539,11 +549,11 @@ __device__ __forceinline__ void quad_label(TileSmem<T>& sm, const uint32_t (&st)
     __syncthreads();
 }
 
-// Stores four mask bytes of a quad: one dword when the quad is whole and the
-// destination aligned, else byte by byte (only the pixels in `sel`).
-__device__ __forceinline__ void store_quad(uint8_t* dst, uint32_t packed, uint32_t sel, bool aligned) {
-    if (sel == 0xFu && aligned) {
-        *reinterpret_cast<uint32_t*>(dst) = packed;
+// Stores four mask bytes of a quad: one dword when the quad is whole, else
+// byte by byte (only the pixels in `sel`).
+__device__ __forceinline__ void store_quad(uint8_t* dst, uint32_t packed, uint32_t sel) {
+    if (sel == 0xFu) {
+        store_u32(dst, packed);
     } else {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -552,9 +562,9 @@ __device__ __forceinline__ void store_quad(uint8_t* dst, uint32_t packed, uint32
 }
 
 // Same with the selection as 0xFF bytes.
-__device__ __forceinline__ void store_bytes(uint8_t* dst, uint32_t packed, uint32_t selb, bool aligned) {
-    if (selb == 0xFFFFFFFFu && aligned) {
-        *reinterpret_cast<uint32_t*>(dst) = packed;
+__device__ __forceinline__ void store_bytes(uint8_t* dst, uint32_t packed, uint32_t selb) {
+    if (selb == 0xFFFFFFFFu) {
+        store_u32(dst, packed);
     } else {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -610,7 +620,6 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
     const bool mine_eq = quad_init<T, NT>(sm, tw, th, st, sb, v0);
     const uint32_t seam_bits = perimeter_pass<T, NT>(sm, im, x0, y0, tw, th);
     const bool uniform = __syncthreads_and(mine_eq) != 0;
-    const bool al_out = (((uintptr_t)im.mx | (uintptr_t)im.mn | (uintptr_t)im.opitch) & 3u) == 0;
     uint8_t* rec = ws.rec + gtile * 4 * T;
 
     if (uniform) {
@@ -636,8 +645,8 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
             Quads<T, NT>::at(k, ly, lx0);
             const uint32_t selb = bytes80((sb[k] << 3) & B_G);
             const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
-            store_bytes(im.mx + o, mxb, selb, al_out);
-            store_bytes(im.mn + o, mnb, selb, al_out);
+            store_bytes(im.mx + o, mxb, selb);
+            store_bytes(im.mn + o, mnb, selb);
         }
         write_perimeter_records<T, NT>(sm, rec, tw, th, seam_bits, true);
     } else {
@@ -693,8 +702,8 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
                 if (!(zf & F_S)) mnw |= rb;
             }
             const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
-            store_bytes(im.mx + o, mxw, selb, al_out);
-            store_bytes(im.mn + o, mnw, selb, al_out);
+            store_bytes(im.mx + o, mxw, selb);
+            store_bytes(im.mn + o, mnw, selb);
             if (open) {
                 const uint32_t closed = REC_CLOSED * 0x01010101u | (mxw & 0x01010101u) | (mnw & 0x02020202u);
                 *reinterpret_cast<uint32_t*>(plane + ly * T + lx0) = (closed & ~om) | ids;
@@ -834,7 +843,6 @@ __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, i
     const int tx = (int)(lt % (uint32_t)im.tiles_x), ty = (int)(lt / (uint32_t)im.tiles_x);
     const int x0 = tx * T, y0 = ty * T;
     const int tw = min(T, im.w - x0), th = min(T, im.h - y0);
-    const bool al_out = (((uintptr_t)im.mx | (uintptr_t)im.mn | (uintptr_t)im.opitch) & 3u) == 0;
     if (cw & CNT_UNIFORM) {
         const uint32_t v = lut[0];
         const uint32_t mxb = (v & 1u) ? 0xFFFFFFFFu : 0u, mnb = (v & 2u) ? 0xFFFFFFFFu : 0u;
@@ -843,8 +851,8 @@ __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, i
             if (ly >= th || lx0 >= tw) continue;
             const uint32_t sel = tw - lx0 >= 4 ? 0xFu : ((1u << (tw - lx0)) - 1u);
             const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
-            store_quad(im.mx + o, mxb, sel, al_out);
-            store_quad(im.mn + o, mnb, sel, al_out);
+            store_quad(im.mx + o, mxb, sel);
+            store_quad(im.mn + o, mnb, sel);
         }
         return true;
     }
@@ -856,7 +864,6 @@ __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, i
     // as closed, unchanged pixels.
     const uint8_t* plane = ws.rec8 + gtile * (int64_t)(T * T);
     constexpr int VR = T / 16;   // 16-byte vectors per row
-    const bool al16 = (((uintptr_t)im.mx | (uintptr_t)im.mn | (uintptr_t)im.opitch) & 15u) == 0;
     constexpr uint32_t CL4 = REC_CLOSED * 0x01010101u;
     for (int u = lane; u < T * T / 16; u += 64) {
         const int ly = u / VR, c0 = 16 * (u % VR);
@@ -894,17 +901,17 @@ __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, i
         if (!((bal >> (lane & ~(VR - 1))) & ((1u << VR) - 1u))) continue;
         if (ly >= th || c0 >= tw) continue;
         const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + c0;
-        if (al16 && c0 + 16 <= tw) {
-            *reinterpret_cast<uint4*>(im.mx + o) = make_uint4(mxw[0], mxw[1], mxw[2], mxw[3]);
-            *reinterpret_cast<uint4*>(im.mn + o) = make_uint4(mnw[0], mnw[1], mnw[2], mnw[3]);
+        if (c0 + 16 <= tw) {
+            store_u128(im.mx + o, make_uint4(mxw[0], mxw[1], mxw[2], mxw[3]));
+            store_u128(im.mn + o, make_uint4(mnw[0], mnw[1], mnw[2], mnw[3]));
         } else {
 #pragma unroll
             for (int d = 0; d < 4; ++d) {
                 const int nv = tw - c0 - 4 * d;
                 if (nv <= 0) break;
                 const uint32_t sel = nv >= 4 ? 0xFu : ((1u << nv) - 1u);
-                store_quad(im.mx + o + 4 * d, mxw[d], sel, al_out);
-                store_quad(im.mn + o + 4 * d, mnw[d], sel, al_out);
+                store_quad(im.mx + o + 4 * d, mxw[d], sel);
+                store_quad(im.mn + o + 4 * d, mnw[d], sel);
             }
         }
     }
