@@ -88,10 +88,14 @@ def batched_leg(n, dev, ss, ssgen, torch, reps=3):
     step = max(1, n // 64)
     sample = list(range(0, n, step))
     sample_descs = ss.descs([imgs[i] for i in sample], [outs[i] for i in sample])
-    # each path with the configuration its own autotuner picks on the sample:
-    # per-image launches by the per-image rule, the batch by batched launches
+    # each path with the configuration its own autotuner picks on a sample:
+    # per-image launches by the per-image rule on 64 images, the batch by
+    # batched launches on 512 (a smaller batch is launch-bound, and its
+    # ranking of the candidates says little about the full one)
     (ptile, pthreads), _ = ss.autotune(sample_descs, reps=2)
-    (tile, threads), _ = ss.autotune(sample_descs, reps=2, batch=True)
+    bsample = list(range(0, n, max(1, n // 512)))
+    bsample_descs = ss.descs([imgs[i] for i in bsample], [outs[i] for i in bsample])
+    (tile, threads), _ = ss.autotune(bsample_descs, reps=3, batch=True)
     log("batch: autotuned per-image tile=%d threads=%d, batched tile=%d threads=%d" % (ptile, pthreads, tile, threads))
     stream = torch.cuda.current_stream()
     ev = lambda: torch.cuda.Event(enable_timing=True)
