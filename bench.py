@@ -248,7 +248,7 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py runs on one GPU only: images are independent and this build has no "
+        print("bench.py runs on one GPU only: images are independent and this library has no "
               "multi-GPU path (sharding would assign whole images, or tile-row bands with one "
               "seam-fragment all-gather, per GPU).", file=sys.stderr)
         sys.exit(2)

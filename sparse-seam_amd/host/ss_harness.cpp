@@ -452,7 +452,7 @@ int main(int argc, char** argv) {
                "tile,active_fragments,dirty_tiles,dsu_retries\n";
         // per-image validation rows with the columns of the reference's
         // snapshot harness (snapshot_sparse_seam_cuda_v1_0.cu:931-933,1000-1014);
-        // auxiliary bytes are this build's workspace (the masks and the image
+        // auxiliary bytes are the library's workspace (the masks and the image
         // are the caller's)
         std::ofstream vcsv(out_dir / "validation.csv");
         vcsv << "image,width,height,tiles,fragment_capacity,active_fragments,"

@@ -158,7 +158,7 @@ def test_full_size_sweep_vs_global_uf(dev):
     """The benchmark's workload at full size and its default configuration
     (T = 64, 256 threads): every image of the 16384x16384 sweep, bit-exact
     against the CPU Global-UF route (one image is a second or two of CPU
-    work on the box's cores)."""
+    work on the host's cores)."""
     ws = None
     for name, img in ssgen.sweep_images(16384, 0, device="cuda"):
         say("full-size sweep 16384 %s" % name)
