@@ -75,7 +75,8 @@ ss_status ss_workspace_size(int width, int height, const ss_config* cfg, size_t*
 /* Exact dual regional extrema of one image.  Replaces run_sparse_seam /
  * run_sparse_resident (snapshot_sparse_seam_cuda_v1_0.cu:753-878,
  * gpu_common_v0_6.cu:137-206) with the image already resident in HBM.
- *   d_img    : width x height uint8 pixels, row r at d_img + r*pitch (pitch >= width)
+ *   d_img    : width x height uint8 pixels, row r at d_img + r*pitch (pitch >= width;
+ *              any pitch and base address, no alignment required)
  *   d_max/min: outputs, row r at d + r*out_pitch (out_pitch >= width); both masks
  *              are fully written (every pixel 0 or 255)
  *   d_ws     : workspace of ws_bytes >= ss_workspace_size(...), 256-byte aligned
