@@ -10,8 +10,10 @@ import sys
 import numpy as np
 import pytest
 import torch
+from hypothesis import HealthCheck, given, settings
+from hypothesis import strategies as hst
 
-from sscommon import golden_cases, oracle_global_uf, oracle_sparse_seam
+from sscommon import golden_cases, oracle_bruteforce, oracle_global_uf, oracle_quantize, oracle_sparse_seam
 from ssx import sparse_seam as ss
 from ssx import ssgen
 
@@ -70,6 +72,24 @@ def test_random_images_all_configs(dev, seed):
         mx, mn, _, _ = run(img, tile, threads)
         assert np.array_equal(mx, emax), (tile, threads)
         assert np.array_equal(mn, emin), (tile, threads)
+
+
+@settings(max_examples=40, deadline=None, derandomize=True,
+          suppress_health_check=[HealthCheck.too_slow, HealthCheck.function_scoped_fixture])
+@given(h=hst.integers(1, 130), w=hst.integers(1, 130), levels=hst.sampled_from([2, 3, 4, 16, 256]),
+       block=hst.integers(1, 12), seed=hst.integers(0, 2**32 - 1),
+       config=hst.sampled_from(list(ss.CONFIGS)))
+def test_property_small_images(dev, h, w, levels, block, seed, config):
+    """Property form of the HIP-vs-oracle check (sizes 1..130 straddling the
+    tile sides, blocky quantised content so that zones cross seams): the C-ABI
+    path at a drawn (T, threads) equals the brute-force closed form."""
+    rng = np.random.default_rng(seed)
+    small = rng.integers(0, 256, (-(-h // block), -(-w // block)), dtype=np.uint8)
+    img = np.ascontiguousarray(np.repeat(np.repeat(small, block, 0), block, 1)[:h, :w])
+    img = oracle_quantize(img, levels)
+    emax, emin = oracle_bruteforce(img)
+    mx, mn, _, _ = run(img, *config)
+    assert np.array_equal(mx, emax) and np.array_equal(mn, emin), (h, w, config)
 
 
 @pytest.mark.parametrize("tile", ss.TILES)

@@ -2,6 +2,8 @@
 closed form, tile-count invariants and special cases.  CPU only."""
 import numpy as np
 import pytest
+from hypothesis import HealthCheck, given, settings
+from hypothesis import strategies as hst
 from scipy import ndimage
 
 from sscommon import (golden_cases, mismatches, oracle_bruteforce, oracle_global_uf,
@@ -113,6 +115,24 @@ def test_routes_agree_on_random_images(seed):
     for tile in (16, 32, 64, int(rng.integers(1, 65))):
         tmx, tmn, _ = oracle_sparse_seam(img, tile)
         assert np.array_equal(bmx, tmx) and np.array_equal(bmn, tmn), tile
+
+
+@settings(max_examples=60, deadline=None, derandomize=True, suppress_health_check=[HealthCheck.too_slow])
+@given(h=hst.integers(1, 130), w=hst.integers(1, 130), levels=hst.sampled_from([2, 3, 4, 8, 16, 256]),
+       kind=hst.sampled_from(["iid", "blocks", "ramp"]), seed=hst.integers(0, 2**32 - 1),
+       tile=hst.integers(1, 64))
+def test_routes_agree_property(h, w, levels, kind, seed, tile):
+    """Property form of the cross-route check (sizes 1..130, any tile side):
+    brute force, Global-UF and Sparse-Seam at T in {16, 32, 64} and one drawn
+    T agree; hypothesis reaches the edge sizes (1, tile multiples +- 1) and
+    shrinks a failure to its smallest image."""
+    img = _random_image(np.random.default_rng(seed), h, w, levels, kind)
+    bmx, bmn = oracle_bruteforce(img)
+    gmx, gmn = oracle_global_uf(img, threads=2)
+    assert np.array_equal(bmx, gmx) and np.array_equal(bmn, gmn)
+    for t in (16, 32, 64, tile):
+        tmx, tmn, _ = oracle_sparse_seam(img, t)
+        assert np.array_equal(bmx, tmx) and np.array_equal(bmn, tmn), t
 
 
 def tile_seam_counts(img, tile):
