@@ -75,8 +75,13 @@ __device__ __forceinline__ RowMinMax row_minmax(uint32_t x0, uint32_t x1, uint32
 template <int T>
 struct TileSmem {
     static constexpr int P = T + 16;  // LDS row pitch in bytes (strips of 4 rows hit disjoint banks)
-    alignas(16) uint8_t img[(T + 2) * P];
-    alignas(16) uint32_t lab[T * T];  // labels; at roots also evidence and fragment id
+    // The staged rows are read only by the classification and the perimeter
+    // pass; the labels are stored after the barrier that follows them, so the
+    // two share their LDS (one more workgroup per CU at T = 64).
+    union {
+        alignas(16) uint8_t img[(T + 2) * P];
+        alignas(16) uint32_t lab[T * T];  // labels; at roots also evidence and fragment id
+    };
     uint8_t eqr[T];                   // cross-seam equality bits, right column
     uint8_t eqb[T];                   // cross-seam equality bits, bottom row
     uint32_t count;
@@ -393,16 +398,13 @@ __device__ __forceinline__ uint32_t classify_quad(TileSmem<T>& sm, const RowW& a
                      movemask_pair(eul80, eur80) << Q_EUL;
         const uint32_t nsg4 = movemask_pair(val80 & ~neq, 0u);
         s |= (nsg4 & ~((s >> Q_EL) & 0xEu)) << Q_HD;
-        // Labels of the quad in one 16-byte store: a head links to its first
-        // contact above (else to the pixel left of the quad, else to itself),
-        // every other run pixel to its head.  Singletons get a dummy entry
-        // that nothing reads.  Every link points at a smaller index.
+        // Union and root-candidate heads of the quad; its labels are stored
+        // by store_labels once the staged rows are no longer read.
         const uint32_t p0 = ly * T + lx0;
         uint32_t lab4[4], un, rt;
         quad_links(s, p0, (uint32_t)T, lab4, un, rt);
         st = s | un << Q_UN;
         rtb = (rt * 0x00204081u) & B_RT;
-        *reinterpret_cast<uint4*>(&sm.lab[p0]) = make_uint4(lab4[0], lab4[1], lab4[2], lab4[3]);
     }
     return g80 | (s80 >> 1) | (sgl80 >> 2) | (val80 >> 3) | rtb;
 }
@@ -442,6 +444,26 @@ __device__ __forceinline__ bool quad_init(TileSmem<T>& sm, int tw, int th, uint3
         }
     }
     return all_eq;
+}
+
+// Komura labels of the thread's non-singleton quads, recomputed from their
+// edge words, one 16-byte store per quad: a head links to its first contact
+// above (else to the pixel left of the quad, else to itself), every other run
+// pixel to its head; a singleton in such a quad gets a dummy entry that nothing
+// reads.  Every link points at a smaller index.  Called once nothing reads the
+// staged rows any more (the labels overwrite them) and followed by a barrier.
+template <int T, int NT>
+__device__ __forceinline__ void store_labels(TileSmem<T>& sm, const uint32_t (&st)[Quads<T, NT>::PER]) {
+#pragma unroll
+    for (int k = 0; k < Quads<T, NT>::PER; ++k) {
+        if (!st[k]) continue;
+        int ly, lx0;
+        Quads<T, NT>::at(k, ly, lx0);
+        const uint32_t p0 = ly * T + lx0;
+        uint32_t lab4[4], un, rt;
+        quad_links(st[k], p0, (uint32_t)T, lab4, un, rt);
+        *reinterpret_cast<uint4*>(&sm.lab[p0]) = make_uint4(lab4[0], lab4[1], lab4[2], lab4[3]);
+    }
 }
 
 // Perimeter slot e in [0, 4T): top row, bottom row, left and right columns
@@ -497,8 +519,8 @@ __device__ __forceinline__ uint32_t perimeter_pass(TileSmem<T>& sm, const ImgDev
 // run: a segment is one zone, its pixels being equal and horizontally
 // adjacent; and the left contact of a run that starts the quad), then a
 // read-only final compression that also ORs each run's evidence, and the seam
-// evidence of the perimeter slots, into the roots.  Assumes a barrier after
-// quad_init; ends with one.
+// evidence of the perimeter slots, into the roots.  Assumes the labels stored
+// (store_labels) and a barrier; ends with one.
 template <int T, int NT>
 __device__ __forceinline__ void quad_label(TileSmem<T>& sm, const uint32_t (&st)[Quads<T, NT>::PER],
                                            const uint32_t (&sb)[Quads<T, NT>::PER], int tw, int th,
@@ -650,6 +672,8 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
         }
         write_perimeter_records<T, NT>(sm, rec, tw, th, seam_bits, true);
     } else {
+        store_labels<T, NT>(sm, st);
+        __syncthreads();
         quad_label<T, NT>(sm, st, sb, tw, th, seam_bits);
         // Number the seam zones (their roots) and fill their fragment slots.
         {
@@ -778,6 +802,9 @@ __device__ __forceinline__ void tile_profile(TileSmem<T>& sm, const ImgDev& im, 
             seam_px = any_seam ? (uint32_t)(tw * th) : 0u;
         }
     } else {
+        __syncthreads();   // the equal-pair counts above read the staged rows
+        store_labels<T, NT>(sm, st);
+        __syncthreads();
         quad_label<T, NT>(sm, st, sb, tw, th, seam_bits);
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
