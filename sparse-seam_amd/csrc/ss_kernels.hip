@@ -555,11 +555,22 @@ __global__ __launch_bounds__(256) void k_writeback(Images ims, Ws ws) {
     const int64_t w = xcd_block(blockIdx.x, gridDim.x) * 4 + wid;
     const int64_t t0 = ws.ntiles * w / nw, t1 = ws.ntiles * (w + 1) / nw;
     uint32_t nd = 0;
-    for (int64_t t = t0; t < t1; ++t) {
-        const uint32_t cw = ws.counts[t];
-        if (!(cw & COUNT_MASK) || !(cw & (CNT_OPEN | CNT_UNIFORM))) continue;
-        const ImgDev im = locate<MULTI>(ims, ws.tile_img, t);
-        nd += tile_writeback<T>(lut[wid], im, t, cw, ws) ? 1u : 0u;
+    const int lane = threadIdx.x & 63;
+    // the counts and base words of up to 64 tiles in one load each; the tiles
+    // that hold a fragment which may change are then taken one by one
+    for (int64_t tb = t0; tb < t1; tb += 64) {
+        const int64_t tl = tb + lane;
+        const uint32_t cwl = tl < t1 ? ws.counts[tl] : 0u;
+        const uint32_t bl = tl < t1 ? ws.base[tl] : 0u;
+        uint64_t todo = __ballot((cwl & COUNT_MASK) && (cwl & (CNT_OPEN | CNT_UNIFORM)));
+        while (todo) {
+            const int i = __ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1;
+            const int64_t t = tb + i;
+            const uint32_t cw = __shfl(cwl, i, 64), b = __shfl(bl, i, 64);
+            const ImgDev im = locate<MULTI>(ims, ws.tile_img, t);
+            nd += tile_writeback<T>(lut[wid], im, t, cw, b, ws) ? 1u : 0u;
+        }
     }
     if ((threadIdx.x & 63) == 0 && nd) atomicAdd(&s_dirty, nd);
     __syncthreads();

@@ -844,13 +844,13 @@ __device__ __forceinline__ void tile_profile(TileSmem<T>& sm, const ImgDev& im, 
 // other tile streams its record plane (fragment ids, or the final masks of
 // closed pixels) and rewrites every whole tile row that holds a changed pixel,
 // so HBM sees full row segments rather than scattered partial lines.  No image
-// read, no relabelling.  Returns whether the tile changed.
+// read, no relabelling.  b is the tile's base (dense id of its first
+// fragment).  Returns whether the tile changed.
 template <int T>
 __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, int64_t gtile, uint32_t cw,
-                                               const Ws& ws) {
+                                               uint32_t b, const Ws& ws) {
     const int lane = threadIdx.x & 63;
     const uint32_t cnt = cw & COUNT_MASK;
-    const uint32_t b = ws.base[gtile];
     bool changed = false;
     for (uint32_t j = lane; j < cnt; j += 64) {
         const uint32_t own = ws.dbits[b + j];
