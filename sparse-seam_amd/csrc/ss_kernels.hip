@@ -550,12 +550,11 @@ __global__ __launch_bounds__(256) void k_writeback(Images ims, Ws ws) {
     __shared__ uint32_t s_dirty;
     if (threadIdx.x == 0) s_dirty = 0;
     __syncthreads();
-    const int wid = threadIdx.x >> 6;
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t nw = (int64_t)gridDim.x * 4;
     const int64_t w = xcd_block(blockIdx.x, gridDim.x) * 4 + wid;
     const int64_t t0 = ws.ntiles * w / nw, t1 = ws.ntiles * (w + 1) / nw;
     uint32_t nd = 0;
-    const int lane = threadIdx.x & 63;
     // the counts and base words of up to 64 tiles in one load each; the tiles
     // that hold a fragment which may change are then taken one by one
     for (int64_t tb = t0; tb < t1; tb += 64) {
@@ -572,7 +571,7 @@ __global__ __launch_bounds__(256) void k_writeback(Images ims, Ws ws) {
             nd += tile_writeback<T>(lut[wid], im, t, cw, b, ws) ? 1u : 0u;
         }
     }
-    if ((threadIdx.x & 63) == 0 && nd) atomicAdd(&s_dirty, nd);
+    if (lane == 0 && nd) atomicAdd(&s_dirty, nd);
     __syncthreads();
     if (threadIdx.x == 0 && s_dirty) atomicAdd(&ws.hdr->dirty_count, s_dirty);
 }
