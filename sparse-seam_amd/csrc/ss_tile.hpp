@@ -195,8 +195,8 @@ __device__ __forceinline__ uint32_t lds32(const uint8_t* p) { return *reinterpre
 // root entry with a mere ancestor.
 __device__ __forceinline__ uint32_t lfind_h(uint32_t* lab, uint32_t x) {
     for (;;) {
+        // at a root x the grandparent is x itself, so one test covers both
         const uint32_t p = lds_load(lab + x) & LMASK;
-        if (p == x) return x;
         const uint32_t gp = lds_load(lab + p) & LMASK;
         if (gp == p) return p;
         __hip_atomic_store(lab + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
