@@ -190,6 +190,23 @@ def test_full_size_sweep_vs_global_uf(dev):
         assert np.array_equal(mn.cpu().numpy(), emin), name
 
 
+def test_many_tiles_per_writeback_wave(dev):
+    """T = 16 at 16384x16384: over a million tiles, so each write-back wave
+    walks its tile range in several chunks of 64 (the counts and base words of
+    a chunk are loaded at once); bit-exact against Global-UF."""
+    name, img = "smooth_s4_q4", None
+    for n, im in ssgen.sweep_images(16384, 0, device="cuda", sigmas=(4,), levels=(4,)):
+        name, img = n, im
+    say("writeback chunks 16384 T=16 %s" % name)
+    mx, mn, ws = ss.extrema(img, 16, 256)
+    torch.cuda.synchronize()
+    st = ss.stats(ws)
+    assert st["invariant"] == 0 and st["tiles"] == 1024 * 1024 and st["dirty_tiles"] > 0
+    emax, emin = oracle_global_uf(img.cpu().numpy())
+    assert np.array_equal(mx.cpu().numpy(), emax)
+    assert np.array_equal(mn.cpu().numpy(), emin)
+
+
 def test_strided_rows_and_outputs(dev):
     rng = np.random.default_rng(9)
     base = torch.as_tensor(rng.integers(0, 4, (130, 300), dtype=np.uint8)).cuda()
