@@ -361,8 +361,9 @@ __device__ __forceinline__ uint32_t first_bytes80(int n) {
 }
 
 // Classifies one quad from its three rows: byte-form evidence (returned), and
-// for a quad holding a non-singleton pixel also the 4-bit edge word of the
-// labelling passes (else 0) and its Komura labels.  Singletons (no equal value among
+// for a quad holding a non-singleton pixel also the edge word of the labelling
+// passes (else 0), from which store_labels later derives its Komura labels.
+// Singletons (no equal value among
 // the eight LDS neighbours, halo included -- conservative at the image border,
 // where a replicated halo value may only make a pixel non-singleton) take no
 // label.  Pixels past tw or th are masked out of every flag.
@@ -410,10 +411,10 @@ __device__ __forceinline__ uint32_t classify_quad(TileSmem<T>& sm, const RowW& a
 }
 
 // First pass over the thread's quads (a vertical strip): evidence, singleton
-// detection, the edge bits of the union passes and the Komura links (each
-// head links to an equal pixel above, else left).  sb[k] gets the byte-form
-// evidence, st[k] the edge word (0 for a quad of singletons).  Returns
-// whether every pixel equals v0.
+// detection and the edge bits of the union passes and of the Komura links
+// (each head links to an equal pixel above, else left; stored later by
+// store_labels).  sb[k] gets the byte-form evidence, st[k] the edge word (0
+// for a quad of singletons).  Returns whether every pixel equals v0.
 template <int T, int NT>
 __device__ __forceinline__ bool quad_init(TileSmem<T>& sm, int tw, int th, uint32_t (&st)[Quads<T, NT>::PER],
                                           uint32_t (&sb)[Quads<T, NT>::PER], uint32_t v0) {
