@@ -368,7 +368,7 @@ __device__ __forceinline__ uint32_t first_bytes80(int n) {
 // where a replicated halo value may only make a pixel non-singleton) take no
 // label.  Pixels past tw or th are masked out of every flag.
 template <int T>
-__device__ __forceinline__ uint32_t classify_quad(TileSmem<T>& sm, const RowW& a, const RowW& b, const RowW& c,
+__device__ __forceinline__ uint32_t classify_quad(const RowW& a, const RowW& b, const RowW& c,
                                                   int ly, int lx0, int tw, int th, uint32_t v4, bool& all_eq,
                                                   uint32_t& st) {
     const uint32_t V = b.w;
@@ -428,7 +428,7 @@ __device__ __forceinline__ bool quad_init(TileSmem<T>& sm, int tw, int th, uint3
         sb[0] = 0;
         if (!has || ly >= th || lx0 >= tw) return true;
         const RowW a = row_at<T>(sm, ly - 1, lx0), b = row_at<T>(sm, ly, lx0), c = row_at<T>(sm, ly + 1, lx0);
-        sb[0] = classify_quad<T>(sm, a, b, c, ly, lx0, tw, th, v4, all_eq, st[0]);
+        sb[0] = classify_quad<T>(a, b, c, ly, lx0, tw, th, v4, all_eq, st[0]);
     } else {
         const int r0 = ly;
         const bool col_ok = lx0 < tw;
@@ -439,7 +439,7 @@ __device__ __forceinline__ bool quad_init(TileSmem<T>& sm, int tw, int th, uint3
             st[k] = 0;
             sb[k] = 0;
             if (col_ok && r0 + k < th)
-                sb[k] = classify_quad<T>(sm, a, b, c, r0 + k, lx0, tw, th, v4, all_eq, st[k]);
+                sb[k] = classify_quad<T>(a, b, c, r0 + k, lx0, tw, th, v4, all_eq, st[k]);
             a = b;
             b = c;
         }
