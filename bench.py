@@ -153,14 +153,10 @@ def serving_leg(n, dev, ss, ssgen, torch, cap_bytes=32 << 30):
     GB.  Plain per-request path (H2D, one launch, D2H, in order) versus the
     continuous-batching server at the plain path's own sustainable rate."""
     import numpy as np
-    rng = np.random.default_rng([0, 9])
     t0 = time.time()
     reqs_np = []
     keep = []
-    for k in range(n):
-        s = float(np.exp(rng.uniform(np.log(64), np.log(8192))))
-        a = float(np.exp(rng.uniform(np.log(0.5), np.log(2.0))))
-        w, h = max(1, int(round(s * a ** 0.5))), max(1, int(round(s / a ** 0.5)))
+    for k, (h, w) in enumerate(ssgen.serving_sizes(n, seed=0)):
         img = ssgen.mixed_image(k, h, w, 1, dev) if k % 3 != 2 else torch.as_tensor(ssgen.edt_disks(h, w, k))
         host = torch.empty((h, w), dtype=torch.uint8, pin_memory=True)
         host.copy_(img)

@@ -196,9 +196,19 @@ void ss_server::schedule_loop() {
                 hipMemcpyAsync(q.r.host_min, base + q.off_min, q.px, hipMemcpyDeviceToHost, st) != hipSuccess)
                 status = SS_ERR_HIP;
         }
-        if (hipEventCreateWithFlags(&L.done, hipEventDisableTiming) != hipSuccess ||
-            hipEventRecord(L.done, st) != hipSuccess)
+        if (hipEventCreateWithFlags(&L.done, hipEventDisableTiming) != hipSuccess) {
+            L.done = nullptr;
             status = SS_ERR_HIP;
+        } else if (hipEventRecord(L.done, st) != hipSuccess) {
+            (void)hipEventDestroy(L.done);
+            L.done = nullptr;
+            status = SS_ERR_HIP;
+        }
+        // Without a recorded completion event the reaper cannot tell when the
+        // launch's copies and kernels end, and it would free their arena
+        // blocks at once: drain the stream here so that nothing still uses
+        // them when the launch is retired.
+        if (!L.done) (void)hipStreamSynchronize(st);
         L.status = status;
         lk.lock();
         stats.launches++;
@@ -293,8 +303,9 @@ ss_status ss_server_create(const ss_server_config* cfg, ss_server** out) {
     s->arena.reset(cfg->hbm_cap_bytes);
     s->streams.resize((size_t)s->cfg.streams);
     s->stream_busy.assign((size_t)s->cfg.streams, 0);
-    for (auto& st : s->streams) {
-        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+    for (size_t i = 0; i < s->streams.size(); ++i) {
+        if (hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking) != hipSuccess) {
+            for (size_t j = 0; j < i; ++j) (void)hipStreamDestroy(s->streams[j]);
             (void)hipFree(s->base);
             delete s;
             return SS_ERR_HIP;

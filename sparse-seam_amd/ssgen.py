@@ -183,21 +183,25 @@ def edt_disks(h, w, seed=0):
     rng = np.random.default_rng([seed, 6])
     want = int(rng.integers(20, 301))
     d2 = np.zeros((h, w), np.int64)
-    placed = []
+    # centres and radii of the placed disks (first n entries)
+    pl = np.zeros((3, want), np.int64)
+    n = 0
     for _ in range(want * 20):
-        if len(placed) >= want:
+        if n >= want:
             break
         r = int(rng.integers(5, 41))
         if 2 * r + 3 > min(h, w):
             continue
         cy = int(rng.integers(r + 1, h - r - 1))
         cx = int(rng.integers(r + 1, w - r - 1))
-        if any((cy - py) ** 2 + (cx - px) ** 2 < (r + pr + 3) ** 2 for py, px, pr in placed):
+        py, px, pr = pl[0, :n], pl[1, :n], pl[2, :n]
+        if n and ((cy - py) ** 2 + (cx - px) ** 2 < (r + pr + 3) ** 2).any():
             continue
-        placed.append((cy, cx, r))
+        pl[:, n] = (cy, cx, r)
+        n += 1
         t, m = _disk_d2(r)
         sl = (slice(cy - r - 1, cy + r + 2), slice(cx - r - 1, cx + r + 2))
-        d2[sl] = np.where(m, t, d2[sl])
+        np.copyto(d2[sl], t, where=m)
     m2 = int(d2.max())
     if m2 == 0:
         return np.zeros((h, w), np.uint8)
@@ -241,3 +245,15 @@ def mixed_image(k, h, w, seed=0, device="cpu"):
     if kind == 1:
         return smooth(h, w, 3.0, 16, s, device)
     return torch.as_tensor(edt_disks(h, w, s)).to(device)
+
+
+def serving_sizes(n, seed=0, lo=64, hi=8192):
+    """(h, w) of n serving requests: side log-uniform in lo..hi, aspect ratio
+    log-uniform in 0.5..2 (w / h), the area kept at side^2."""
+    rng = np.random.default_rng([seed, 9])
+    out = []
+    for _ in range(n):
+        s = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        a = float(np.exp(rng.uniform(np.log(0.5), np.log(2.0))))
+        out.append((max(1, int(round(s / a ** 0.5))), max(1, int(round(s * a ** 0.5)))))
+    return out

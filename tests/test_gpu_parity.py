@@ -121,16 +121,27 @@ def test_config1_image(dev):
         assert np.array_equal(mx, emax) and np.array_equal(mn, emin)
 
 
+def run_dev(img, tile, threads, ws=None):
+    """The C-ABI path on a resident image; masks back on the host."""
+    mx, mn, ws = ss.extrema(img, tile, threads, ws)
+    torch.cuda.synchronize()
+    assert ss.stats(ws)["invariant"] == 0
+    return mx.cpu().numpy(), mn.cpu().numpy(), ws
+
+
+@pytest.mark.parametrize("seed", range(5))
 @pytest.mark.parametrize("side", [512, 1024, 2048])
-def test_sweep_small_sides(dev, side):
-    for seed in (0, 1):
-        for name, img in ssgen.sweep_images(side, seed, device="cuda"):
-            say("sweep side=%d seed=%d %s" % (side, seed, name))
-            emax, emin = oracle_global_uf(img.cpu().numpy())
-            for tile in ss.TILES:
-                mx, mn, _, _ = run(img.cpu().numpy(), tile, 256)
-                assert np.array_equal(mx, emax), (name, tile)
-                assert np.array_equal(mn, emin), (name, tile)
+def test_sweep_small_sides(dev, side, seed):
+    """The resolution / gray-level sweep at seeds 0..4: all 19 images at every
+    tile size, bit-exact against the CPU Global-UF route."""
+    ws = {}
+    for name, img in ssgen.sweep_images(side, seed, device="cuda"):
+        say("sweep side=%d seed=%d %s" % (side, seed, name))
+        emax, emin = oracle_global_uf(img.cpu().numpy())
+        for tile in ss.TILES:
+            mx, mn, ws[tile] = run_dev(img, tile, 256, ws.get(tile))
+            assert np.array_equal(mx, emax), (name, tile)
+            assert np.array_equal(mn, emin), (name, tile)
 
 
 def test_adversarial_4096(dev):
@@ -164,30 +175,32 @@ def test_seam_comb_fragment_capacity(dev):
         assert counts.max() >= 4 * tile - 6
 
 
-def test_large_images_vs_global_uf(dev):
-    for side in (4096, 8192):
-        for name, img in ssgen.sweep_images(side, 0, device="cuda", sigmas=(4,), levels=(2, 16)):
-            say("large side=%d %s" % (side, name))
-            h = img.cpu().numpy()
-            emax, emin = oracle_global_uf(h)
-            mx, mn, _, _ = run(h, 32, 256)
-            assert np.array_equal(mx, emax) and np.array_equal(mn, emin), name
+@pytest.mark.parametrize("side", [4096, 8192])
+def test_large_sweep_vs_global_uf(dev, side):
+    """All 19 sweep images (seed 0) at 4096 and 8192, every tile size."""
+    ws = {}
+    for name, img in ssgen.sweep_images(side, 0, device="cuda"):
+        say("large sweep side=%d %s" % (side, name))
+        emax, emin = oracle_global_uf(img.cpu().numpy())
+        for tile in ss.TILES:
+            mx, mn, ws[tile] = run_dev(img, tile, 256, ws.get(tile))
+            assert np.array_equal(mx, emax), (name, tile)
+            assert np.array_equal(mn, emin), (name, tile)
 
 
-def test_full_size_sweep_vs_global_uf(dev):
-    """The benchmark's workload at full size and its default configuration
-    (T = 64, 256 threads): every image of the 16384x16384 sweep, bit-exact
-    against the CPU Global-UF route (one image is a second or two of CPU
-    work on the host's cores)."""
+@pytest.mark.parametrize("tile", [64, 32, 16])
+def test_full_size_sweep_vs_global_uf(dev, tile):
+    """The benchmark's workload at full size: every image of the
+    16384x16384 sweep at each tile size (T = 64 is the benchmark's default
+    configuration), bit-exact against the CPU Global-UF route (one image is a
+    second or two of CPU work on the host's cores)."""
     ws = None
     for name, img in ssgen.sweep_images(16384, 0, device="cuda"):
-        say("full-size sweep 16384 %s" % name)
-        mx, mn, ws = ss.extrema(img, 64, 256, ws)
-        torch.cuda.synchronize()
-        assert ss.stats(ws)["invariant"] == 0
+        say("full-size sweep 16384 tile=%d %s" % (tile, name))
+        mx, mn, ws = run_dev(img, tile, 256, ws)
         emax, emin = oracle_global_uf(img.cpu().numpy())
-        assert np.array_equal(mx.cpu().numpy(), emax), name
-        assert np.array_equal(mn.cpu().numpy(), emin), name
+        assert np.array_equal(mx, emax), name
+        assert np.array_equal(mn, emin), name
 
 
 def test_many_tiles_per_writeback_wave(dev):
