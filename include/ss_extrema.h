@@ -93,6 +93,19 @@ ss_status ss_extrema_u8(const uint8_t* d_img, size_t pitch, int width, int heigh
  * snapshot_sparse_seam_cuda_v1_0.cu:840-850). */
 ss_status ss_read_stats(const void* d_ws, size_t ws_bytes, ss_stats* out, hipStream_t stream);
 
+/* Co-reference route: the dense Global-UF operator (one union-find node per
+ * pixel, equal left / up-left / up / up-right unions with min-root CAS, full
+ * compression, greater / smaller flags ORed into the roots).  Replaces
+ * run_global_uf (snapshot_gpu_benchmark_globaluf_ole_v0_1.cu:341-385, kernels
+ * :105-259), which the reference's harness runs beside Sparse-Seam to
+ * cross-check every timed image (gpu_common_v0_6.cu:659-669).  Same masks as
+ * ss_extrema_u8; a validator, not tuned for speed.  Images below 2^30 pixels;
+ * the workspace (4 bytes per pixel) must be 256-byte aligned.  Asynchronous. */
+ss_status ss_global_uf_workspace_size(int width, int height, size_t* bytes);
+ss_status ss_global_uf_u8(const uint8_t* d_img, size_t pitch, int width, int height,
+                          uint8_t* d_max, uint8_t* d_min, size_t out_pitch,
+                          void* d_ws, size_t ws_bytes, hipStream_t stream);
+
 /* Per-tile seam-fragment counts of the last run (tiles_y x tiles_x uint32),
  * copied to host memory `out` (synchronous).  The reference reads the same
  * counts back to sum active_fragments (snapshot_sparse_seam_cuda_v1_0.cu:852-863). */

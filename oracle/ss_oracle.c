@@ -448,15 +448,6 @@ void ssor_quantize(uint8_t* img, size_t n, int levels)
     }
 }
 
-/* Mask comparison as the reference gate counts it
- * (snapshot_sparse_seam_cuda_v1_0.cu:880-895): pixels where (a != 0) != (b != 0). */
-uint64_t ssor_count_mismatch(const uint8_t* a, const uint8_t* b, size_t n)
-{
-    uint64_t m = 0;
-    for (size_t i = 0; i < n; ++i) m += ((a[i] != 0) != (b[i] != 0));
-    return m;
-}
-
 /* ------------------------------------------------------------------------ */
 /* Seam-sparsity counters (gpu_seam_sparsity_profiler_v0_1.cu:133-294).     */
 /* ------------------------------------------------------------------------ */

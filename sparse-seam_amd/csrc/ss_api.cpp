@@ -22,19 +22,18 @@ ss_config resolve_config(const ss_config* cfg) {
 }
 
 int device_cus() {
-    static std::mutex mu;
-    static std::vector<int> cache;
+    // a fixed table: no host allocation on the launch path
+    static std::atomic<int> cache[MAX_DEVICES];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 256;
-    std::lock_guard<std::mutex> lk(mu);
-    if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
-    if (!cache[dev]) {
-        int cus = 0;
+    if (dev < 0 || dev >= MAX_DEVICES) return 256;
+    int cus = cache[dev].load(std::memory_order_relaxed);
+    if (!cus) {
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             cus = 256;
-        cache[dev] = cus;
+        cache[dev].store(cus, std::memory_order_relaxed);
     }
-    return cache[dev];
+    return cus;
 }
 
 ProfileHook& profile_hook() {
@@ -43,6 +42,27 @@ ProfileHook& profile_hook() {
 }
 
 static bool aligned256(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 255u) == 0; }
+
+ss_status extrema_single(const uint8_t* d_img, size_t pitch, int width, int height, uint8_t* d_max,
+                         uint8_t* d_min, size_t out_pitch, void* d_ws, size_t ws_bytes, const ss_config* cfg,
+                         hipStream_t stream, size_t* need) {
+    if (!d_img || !d_max || !d_min || width <= 0 || height <= 0) return SS_ERR_ARG;
+    if (pitch < (size_t)width || out_pitch < (size_t)width) return SS_ERR_ARG;
+    const ss_config c = resolve_config(cfg);
+    if (!config_valid(c.tile, c.threads)) return SS_ERR_ARG;
+    const int64_t nt = tiles_of(width, height, c.tile);
+    if (nt * (4 * c.tile - 4) > (int64_t)0xFFFFFFF0u) return SS_ERR_ARG;
+    Images ims{};
+    ims.one = make_imgdev(d_img, d_max, d_min, (int64_t)pitch, (int64_t)out_pitch, width, height, c.tile, 0, 0, 0);
+    const Layout L = make_layout(nt, blocks_of(ims.one), seams_of(ims.one), c.tile, 1);
+    if (need) *need = L.bytes;
+    if (!d_ws || ws_bytes < L.bytes || !aligned256(d_ws)) return SS_ERR_WORKSPACE;
+    ims.table = nullptr;
+    ims.count = 1;
+    const Ws ws = make_ws(d_ws, L, c.threads);
+    const hipError_t e = launch_pipeline(ims, ws, c.tile, c.threads, device_cus(), stream);
+    return e == hipSuccess ? SS_OK : SS_ERR_HIP;
+}
 
 }  // namespace ssx
 
@@ -102,21 +122,8 @@ ss_status ss_workspace_size(int width, int height, const ss_config* cfg, size_t*
 ss_status ss_extrema_u8(const uint8_t* d_img, size_t pitch, int width, int height, uint8_t* d_max,
                         uint8_t* d_min, size_t out_pitch, void* d_ws, size_t ws_bytes,
                         const ss_config* cfg, hipStream_t stream) {
-    if (!d_img || !d_max || !d_min || width <= 0 || height <= 0) return SS_ERR_ARG;
-    if (pitch < (size_t)width || out_pitch < (size_t)width) return SS_ERR_ARG;
-    const ss_config c = resolve_config(cfg);
-    if (!config_valid(c.tile, c.threads)) return SS_ERR_ARG;
-    const int64_t nt = tiles_of(width, height, c.tile);
-    if (nt * (4 * c.tile - 4) > (int64_t)0xFFFFFFF0u) return SS_ERR_ARG;
-    Images ims{};
-    ims.one = make_imgdev(d_img, d_max, d_min, (int64_t)pitch, (int64_t)out_pitch, width, height, c.tile, 0, 0, 0);
-    const Layout L = make_layout(nt, blocks_of(ims.one), seams_of(ims.one), c.tile, 1);
-    if (!d_ws || ws_bytes < L.bytes || !aligned256(d_ws)) return SS_ERR_WORKSPACE;
-    ims.table = nullptr;
-    ims.count = 1;
-    const Ws ws = make_ws(d_ws, L, c.threads);
-    const hipError_t e = launch_pipeline(ims, ws, c.tile, c.threads, device_cus(), stream);
-    return e == hipSuccess ? SS_OK : SS_ERR_HIP;
+    return extrema_single(d_img, pitch, width, height, d_max, d_min, out_pitch, d_ws, ws_bytes, cfg, stream,
+                          nullptr);
 }
 
 ss_status ss_read_stats(const void* d_ws, size_t ws_bytes, ss_stats* out, hipStream_t stream) {

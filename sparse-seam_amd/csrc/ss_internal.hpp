@@ -128,6 +128,9 @@ inline int64_t seams_of(const ImgDev& d) {
 // Resolved config: cfg, else the process default.
 ss_config resolve_config(const ss_config* cfg);
 
+// Devices whose per-device state the library caches.
+constexpr int MAX_DEVICES = 64;
+
 // Compute units of the current device (cached per device).
 int device_cus();
 
@@ -139,8 +142,20 @@ struct ProfileHook {
 };
 ProfileHook& profile_hook();
 
+// The single-image operator on a given workspace (ss_extrema_u8's body).  If
+// `need` is not null it receives the workspace bytes the call requires, also
+// when the call fails for lack of them.
+ss_status extrema_single(const uint8_t* d_img, size_t pitch, int width, int height, uint8_t* d_max,
+                         uint8_t* d_min, size_t out_pitch, void* d_ws, size_t ws_bytes, const ss_config* cfg,
+                         hipStream_t stream, size_t* need);
+
 // Enqueues the whole pipeline on `s`.
 hipError_t launch_pipeline(const Images& ims, const Ws& ws, int T, int NT, int cus, hipStream_t s);
+
+// Adds the digest of a w x h byte image (salted by `salt`) to *d_out (device
+// memory, cleared by the caller).  Used to compare candidates' masks.
+hipError_t launch_digest(const uint8_t* p, int64_t pitch, int w, int h, uint64_t salt, uint64_t* d_out,
+                         hipStream_t s);
 
 // Enqueues the seam-sparsity profile of one image (counters zeroed first, in
 // WsHeader::profile).

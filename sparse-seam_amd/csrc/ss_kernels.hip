@@ -510,9 +510,10 @@ __global__ __launch_bounds__(SEAM_THREADS) void k_seam_merge(Images ims, Ws ws) 
 __global__ __launch_bounds__(256) void k_resolve(Ws ws) {
     const uint32_t total = ws.hdr->total;
     const int lane = threadIdx.x & 63;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t wbase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < total; wbase += stride) {
-        const uint32_t i = wbase + lane;
+    // 64-bit position: a total near 2^32 must not wrap the loop counter
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t wbase = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < total; wbase += stride) {
+        const uint32_t i = (uint32_t)wbase + lane;
         uint32_t r = 0, lb = 0;
         const uint32_t d = i < total ? ws.dbits[i] : 0u;
         if (d & DB_BROOT) {
@@ -601,6 +602,50 @@ hipError_t launch_profile(const Images& ims, const Ws& ws, int T, hipStream_t s)
     case 64: k_seam_profile<64><<<grid, 256, 0, s>>>(ims, ws); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+// ---- mask digest (autotuner admission) ------------------------------------
+
+// 64-bit finaliser (an avalanche mix: every input bit reaches every output bit).
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x ^= x >> 31;
+    x *= 0x7FB5D329728EA185ull;
+    x ^= x >> 27;
+    x *= 0x81DADEF4BC2DD44Dull;
+    x ^= x >> 33;
+    return x;
+}
+
+// Order-independent digest of a w x h byte image: the sum over 8-byte chunks
+// of mix64(chunk ^ mix64(salt + chunk position)), the position taken in the
+// packed w x h layout (the pitch does not enter).  Equal images give equal
+// digests; one differing byte changes the digest except with probability
+// about 2^-64.  One thread per chunk, one 64-bit atomic per wave.
+__global__ __launch_bounds__(256) void k_mask_digest(const uint8_t* p, int64_t pitch, int w, int h, uint64_t salt,
+                                                     unsigned long long* out) {
+    const int64_t cpr = (w + 7) / 8;   // chunks per row
+    const int64_t n = cpr * h;
+    uint64_t acc = 0;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t y = c / cpr;
+        const int x0 = (int)(c - y * cpr) * 8;
+        const uint8_t* row = p + y * pitch + x0;
+        uint64_t v = 0;
+        const int nb = min(8, w - x0);
+        for (int b = 0; b < nb; ++b) v |= (uint64_t)row[b] << (8 * b);
+        acc += mix64(v ^ mix64(salt + (uint64_t)c));
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
+    if ((threadIdx.x & 63) == 0 && acc) atomicAdd(out, (unsigned long long)acc);
+}
+
+hipError_t launch_digest(const uint8_t* p, int64_t pitch, int w, int h, uint64_t salt, uint64_t* d_out,
+                         hipStream_t s) {
+    const int64_t chunks = (int64_t)((w + 7) / 8) * h;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((chunks + 255) / 256, 4096));
+    k_mask_digest<<<grid, 256, 0, s>>>(p, pitch, w, h, salt, reinterpret_cast<unsigned long long*>(d_out));
     return hipGetLastError();
 }
 

@@ -72,6 +72,17 @@ inline int run_sparse_resident(DeviceBuffers& b, hipStream_t stream = nullptr) {
     return 1;
 }
 
+// Co-reference route on the image resident in b.image: the dense Global-UF
+// operator writes its masks into b.maxima / b.minima, using its own
+// workspace `ws` of ss_global_uf_workspace_size bytes.  Mirrors run_global_uf
+// (snapshot_gpu_benchmark_globaluf_ole_v0_1.cu:341-385), which returns 1.
+inline int run_global_uf(DeviceBuffers& b, void* ws, size_t ws_bytes, hipStream_t stream = nullptr) {
+    check(ss_global_uf_u8(b.image, (size_t)b.width, b.width, b.height, b.maxima, b.minima, (size_t)b.width, ws,
+                          ws_bytes, stream),
+          "ss_global_uf_u8");
+    return 1;
+}
+
 // Reads the run's counters; throws on a device invariant failure.
 inline ss_stats read_stats(const DeviceBuffers& b, hipStream_t stream = nullptr) {
     ss_stats st{};

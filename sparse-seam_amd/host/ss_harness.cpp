@@ -12,7 +12,11 @@
 //              [--warmup=5] [--reps=30] [--tile=T --threads=N | --autotune]
 //              [--levels=L] [--device=D] [--skip-expected]
 //
-// Exit codes: 0 all masks match, 1 mismatches, 2 usage, 3 error.
+// Every image is also run through the dense Global-UF route
+// (ss_global_uf_u8) and its masks compared with the operator's: the
+// cross_*_mismatches columns (gpu_common_v0_6.cu:659-669).
+// Exit codes: 0 all masks match (oracle and cross-route), 1 mismatches,
+// 2 usage, 3 error.
 //
 //   ss_harness --convert <image_dir> --output=DIR
 //
@@ -472,10 +476,12 @@ int main(int argc, char** argv) {
                 << "levels=" << levels << '\n'
                 << "timing_compute=preallocated,resident_input,task_complete\n"
                 << "timing_h2h=reused_allocation,H2D+compute+D2H_two_masks\n"
-                << "auxiliary_definition=workspace_bytes(all device buffers except input and two masks)\n";
+                << "auxiliary_definition=workspace_bytes(all device buffers except input and two masks)\n"
+                << "cross_route=Global-UF (ss_global_uf_u8) on every image, masks compared byte for byte\n"
+                << "iterations=value returned by run_sparse_resident per timed run\n";
         }
 
-        uint64_t total_mismatch = 0;
+        uint64_t total_mismatch = 0, total_cross = 0;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         SS_HIP_CHECK(hipEventCreate(&e0));
         SS_HIP_CHECK(hipEventCreate(&e1));
@@ -490,10 +496,10 @@ int main(int argc, char** argv) {
                 sparse_seam::run_sparse_resident(b);
                 SS_HIP_CHECK(hipDeviceSynchronize());
             }
-            std::vector<double> compute, h2h;
+            std::vector<double> compute, h2h, iterations;
             for (int i = 0; i < reps; ++i) {
                 SS_HIP_CHECK(hipEventRecord(e0, nullptr));
-                sparse_seam::run_sparse_resident(b);
+                iterations.push_back((double)sparse_seam::run_sparse_resident(b));
                 SS_HIP_CHECK(hipEventRecord(e1, nullptr));
                 SS_HIP_CHECK(hipEventSynchronize(e1));
                 float ms = 0.f;
@@ -510,6 +516,24 @@ int main(int argc, char** argv) {
                 h2h.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
             }
             const ss_stats st = sparse_seam::read_stats(b);
+            // cross-route check against the dense Global-UF route on the same
+            // resident image (gpu_common_v0_6.cu:659-669)
+            uint64_t cross_max = 0, cross_min = 0;
+            {
+                size_t gws = 0;
+                sparse_seam::check(ss_global_uf_workspace_size(img.width, img.height, &gws),
+                                   "ss_global_uf_workspace_size");
+                void* g = nullptr;
+                SS_HIP_CHECK(hipMalloc(&g, gws));
+                std::vector<uint8_t> gx(px), gn(px);
+                sparse_seam::run_global_uf(b, g, gws);
+                SS_HIP_CHECK(hipMemcpy(gx.data(), b.maxima, px, hipMemcpyDeviceToHost));
+                SS_HIP_CHECK(hipMemcpy(gn.data(), b.minima, px, hipMemcpyDeviceToHost));
+                SS_HIP_CHECK(hipFree(g));
+                cross_max = compare_masks(gx, mx);
+                cross_min = compare_masks(gn, mn);
+            }
+            total_cross += cross_max + cross_min;
             uint64_t mm_max = 0, mm_min = 0;
             const std::string stem = files[k].stem().string();
             if (!skip_expected) {
@@ -529,8 +553,9 @@ int main(int argc, char** argv) {
                 << reps << ',' << cfg.threads << ',' << total_bytes << ',' << (double)total_bytes / px << ','
                 << b.workspace_bytes << ',' << (double)b.workspace_bytes / px << ',' << med_c << ','
                 << percentile(compute, 0.95) << ',' << mp / (med_c * 1e-3) << ',' << med_h << ','
-                << percentile(h2h, 0.95) << ',' << mp / (med_h * 1e-3) << ",0,0," << mm_max << ',' << mm_min
-                << ",0,0," << cfg.tile << ',' << st.fragments << ',' << st.dirty_tiles << ',' << st.dsu_retries
+                << percentile(h2h, 0.95) << ',' << mp / (med_h * 1e-3) << ',' << percentile(iterations, 0.5) << ','
+                << percentile(iterations, 0.95) << ',' << mm_max << ',' << mm_min << ',' << cross_max << ','
+                << cross_min << ',' << cfg.tile << ',' << st.fragments << ',' << st.dirty_tiles << ',' << st.dsu_retries
                 << '\n';
             csv.flush();
             const uint64_t tiles = (uint64_t)((img.width + cfg.tile - 1) / cfg.tile) *
@@ -544,15 +569,15 @@ int main(int argc, char** argv) {
             std::cout << "[SS-HARNESS] " << files[k].filename().string() << ' ' << img.width << 'x'
                       << img.height << " compute_ms=" << med_c << " h2h_ms=" << med_h
                       << " fragments=" << st.fragments << " max_mismatch=" << mm_max
-                      << " min_mismatch=" << mm_min << '\n';
+                      << " min_mismatch=" << mm_min << " cross_mismatch=" << cross_max + cross_min << '\n';
             sparse_seam::free_buffers(b);
         }
         SS_HIP_CHECK(hipEventDestroy(e0));
         SS_HIP_CHECK(hipEventDestroy(e1));
         std::cout << (skip_expected ? "[SS-HARNESS-MEASUREMENT-OK]" : "[SS-HARNESS-VALIDATION-OK]")
                   << " images=" << files.size() << " mismatches=" << total_mismatch
-                  << " output=" << out_dir.string() << '\n';
-        return total_mismatch == 0 ? 0 : 1;
+                  << " cross_mismatches=" << total_cross << " output=" << out_dir.string() << '\n';
+        return total_mismatch == 0 && total_cross == 0 ? 0 : 1;
     } catch (const std::exception& e) {
         std::cerr << "[ERROR] " << e.what() << '\n';
         return 3;

@@ -75,7 +75,9 @@ def _lib():
     lib.ss_profile_events.argtypes = [vp, vp]
     lib.ss_set_default_config.argtypes = [cfgp]
     lib.ss_get_default_config.argtypes = [cfgp]
-    for name in ("ss_workspace_size", "ss_extrema_u8", "ss_read_stats", "ss_read_tile_counts",
+    lib.ss_global_uf_workspace_size.argtypes = [i, i, ctypes.POINTER(sz)]
+    lib.ss_global_uf_u8.argtypes = [vp, sz, i, i, vp, vp, sz, vp, sz, vp]
+    for name in ("ss_global_uf_workspace_size", "ss_global_uf_u8", "ss_workspace_size", "ss_extrema_u8", "ss_read_stats", "ss_read_tile_counts",
                  "ss_batch_workspace_size", "ss_extrema_batch_u8", "ss_batch_plan_create",
                  "ss_batch_plan_run", "ss_batch_plan_destroy", "ss_autotune", "ss_autotune_batch",
                  "ss_set_default_config", "ss_get_default_config", "ss_profile_events"):
@@ -154,6 +156,30 @@ def extrema(img, tile=None, threads=None, ws=None, out=None, stream=None):
     _check(_lib().ss_extrema_u8(img.data_ptr(), img.stride(0), w, h, mx.data_ptr(), mn.data_ptr(),
                                 mx.stride(0), ws.data_ptr(), ws.numel(), _cfg(tile, threads),
                                 _stream_handle(stream)), "ss_extrema_u8")
+    return mx, mn, ws
+
+
+def global_uf(img, ws=None, out=None, stream=None):
+    """The dense Global-UF route (ss_global_uf_u8) on a 2-D uint8 CUDA tensor:
+    the co-reference route the harness cross-checks the operator against.
+    Returns (max_mask, min_mask, ws)."""
+    import torch
+    if img.dim() != 2 or img.dtype != torch.uint8 or not img.is_cuda or img.stride(1) != 1:
+        raise ValueError("img must be a 2-D row-major uint8 CUDA tensor")
+    h, w = img.shape
+    need = ctypes.c_size_t()
+    _check(_lib().ss_global_uf_workspace_size(w, h, ctypes.byref(need)), "ss_global_uf_workspace_size")
+    if ws is None or ws.numel() < need.value:
+        ws = new_workspace(need.value, img.device)
+    if out is None:
+        mx = torch.empty((h, w), dtype=torch.uint8, device=img.device)
+        mn = torch.empty((h, w), dtype=torch.uint8, device=img.device)
+    else:
+        mx, mn = out
+    assert mx.stride() == mn.stride() and mx.stride(1) == 1
+    _check(_lib().ss_global_uf_u8(img.data_ptr(), img.stride(0), w, h, mx.data_ptr(), mn.data_ptr(),
+                                  mx.stride(0), ws.data_ptr(), ws.numel(), _stream_handle(stream)),
+           "ss_global_uf_u8")
     return mx, mn, ws
 
 

@@ -76,8 +76,6 @@ def oracle():
                                      ctypes.POINTER(ctypes.c_uint16)]
     lib.ssor_quantize.argtypes = [u8p, ctypes.c_size_t, ctypes.c_int]
     lib.ssor_quantize.restype = None
-    lib.ssor_count_mismatch.argtypes = [u8p, u8p, ctypes.c_size_t]
-    lib.ssor_count_mismatch.restype = ctypes.c_uint64
     lib.ssor_seam_profile.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_uint64)]
     _oracle = lib

@@ -243,3 +243,36 @@ def test_repeat_runs_are_identical(dev):
         if ref is None:
             ref = cur
         assert np.array_equal(ref[0], cur[0]) and np.array_equal(ref[1], cur[1])
+
+
+def test_global_uf_route_golden_and_sweep(dev):
+    """The dense Global-UF route (ss_global_uf_u8, the harness's cross-route)
+    equals the golden triples and the oracle on sweep images, including
+    strided rows and outputs."""
+    ws = None
+    for suite, name, img, emax, emin, _ in CASES:
+        mx, mn, ws = ss.global_uf(torch.as_tensor(img).cuda(), ws)
+        torch.cuda.synchronize()
+        assert np.array_equal(mx.cpu().numpy(), emax), (suite, name)
+        assert np.array_equal(mn.cpu().numpy(), emin), (suite, name)
+    for name, img in ssgen.sweep_images(2048, 3, device="cuda"):
+        say("global-uf route 2048 %s" % name)
+        mx, mn, ws = ss.global_uf(img, ws)
+        emax, emin = oracle_global_uf(img.cpu().numpy())
+        assert np.array_equal(mx.cpu().numpy(), emax), name
+        assert np.array_equal(mn.cpu().numpy(), emin), name
+    for name, img in ssgen.adversarial_images(2048, 32, 0, device="cuda"):
+        say("global-uf route adversarial 2048 %s" % name)
+        mx, mn, ws = ss.global_uf(img, ws)
+        emax, emin = oracle_global_uf(img.cpu().numpy())
+        assert np.array_equal(mx.cpu().numpy(), emax), name
+        assert np.array_equal(mn.cpu().numpy(), emin), name
+    base = ssgen.smooth(300, 500, 2.0, 4, 1, device="cuda")
+    view = base[:, 13:13 + 401]
+    omx = torch.full((300, 480), 7, dtype=torch.uint8, device="cuda")
+    omn = torch.full((300, 480), 7, dtype=torch.uint8, device="cuda")
+    mx, mn, _ = ss.global_uf(view, out=(omx[:, :401], omn[:, :401]))
+    torch.cuda.synchronize()
+    emax, emin = oracle_global_uf(view.cpu().numpy())
+    assert np.array_equal(mx.cpu().numpy(), emax) and np.array_equal(mn.cpu().numpy(), emin)
+    assert (omx[:, 401:] == 7).all() and (omn[:, 401:] == 7).all()

@@ -47,7 +47,11 @@ def test_golden_gate(tmp_path, suite):
     rows = list(csv.DictReader(open(out / "ss_harness.csv")))
     assert len(rows) == len(os.listdir(os.path.join(root, "inputs")))
     assert all(row["oracle_max_mismatches"] == "0" and row["oracle_min_mismatches"] == "0" for row in rows)
-    assert (out / "environment.txt").exists()
+    # the cross-route columns come from a real Global-UF run on every image
+    assert all(row["cross_max_mismatches"] == "0" and row["cross_min_mismatches"] == "0" for row in rows)
+    assert all(float(row["median_iterations"]) == 1.0 for row in rows)
+    assert "cross_mismatch=0" in r.stdout and "cross_mismatches=0" in r.stdout
+    assert "cross_route=Global-UF" in (out / "environment.txt").read_text()
     val = list(csv.DictReader(open(out / "validation.csv")))
     assert list(val[0].keys()) == ["image", "width", "height", "tiles", "fragment_capacity", "active_fragments",
                                    "auxiliary_bytes", "auxiliary_bytes_per_pixel", "elapsed_ms", "oracle_checked",

@@ -35,3 +35,17 @@ def test_seam_unit_order(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "4096 grids, 0 failed" in r.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_autotune_admission_rule(tmp_path):
+    """Host check of the autotuner's admission rule (ss_admit.hpp) over every
+    ok / digest assignment of up to six candidates."""
+    exe = tmp_path / "admit_check"
+    src = os.path.join(REPO, "tests", "admit_check.cpp")
+    inc = os.path.join(REPO, "sparse-seam_amd", "csrc")
+    subprocess.run([HIPCC, "-x", "hip", "--offload-host-only", "-O2", "-std=c++17", "-I", inc, "-o", str(exe), src],
+                   check=True, capture_output=True, text=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 failed" in r.stdout
