@@ -87,6 +87,23 @@ ss_status ss_extrema_u8(const uint8_t* d_img, size_t pitch, int width, int heigh
                         void* d_ws, size_t ws_bytes, const ss_config* cfg,
                         hipStream_t stream);
 
+/* Allocation-free form of ss_extrema_u8.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * workspace is a library-owned device array of SS_NOALLOC_SCRATCH_BYTES per GPU
+ * (held until the process ends); images whose workspace
+ * (ss_workspace_size) exceeds it are refused with SS_ERR_CAPACITY (it holds
+ * 4096 x 4096 at every tile size, 6144 x 6144 at T = 64).
+ * The call allocates nothing.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (several streams or threads) race and must be
+ * ordered by the caller.  d_invariant (optional, device memory) receives the
+ * run's invariant code (0 = ok; see ss_stats.invariant), since the workspace
+ * is not the caller's to pass to ss_read_stats.  Asynchronous on `stream`. */
+#define SS_NOALLOC_SCRATCH_BYTES (64u << 20)
+ss_status ss_extrema_u8_noalloc(const uint8_t* d_img, size_t pitch, int width, int height,
+                                uint8_t* d_max, uint8_t* d_min, size_t out_pitch,
+                                const ss_config* cfg, uint32_t* d_invariant, hipStream_t stream);
+
 /* Synchronises `stream` and reads the counters of the last run that used d_ws.
  * Returns SS_ERR_INVARIANT when the device invariant word is non-zero (the
  * reference throws "Sparse-seam device invariant failed",
