@@ -41,15 +41,15 @@ def percentile(values, q):
     return v[lo] + (v[hi] - v[lo]) * (pos - lo)
 
 
-def cpu_baseline(images, threads, runs=2):
-    """Global-UF oracle route on the host cores over a bounded sample."""
+def cpu_baseline(images, threads, runs=3):
+    """Global-UF oracle route on the host cores: one warm-up run (page faults,
+    thread pool), then `runs` timed runs of every image; sum(px) / sum(median)."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from sscommon import oracle_global_uf
     px, med = 0, 0.0
     names = []
-    for name, img in images:
-        h = img.cpu().numpy()
-        oracle_global_uf(h, threads)  # warm-up (page faults, thread pool)
+    oracle_global_uf(images[0][1], threads)
+    for name, h in images:
         ts = []
         for _ in range(runs):
             t0 = time.perf_counter()
@@ -99,24 +99,14 @@ def batched_leg(n, dev, ss, ssgen, torch, reps=3):
     log("batch: autotuned per-image tile=%d threads=%d, batched tile=%d threads=%d" % (ptile, pthreads, tile, threads))
     stream = torch.cuda.current_stream()
     ev = lambda: torch.cuda.Event(enable_timing=True)
-    # per-image launches of the plain path
+    # per-image launches of the plain path, timed from a C++ loop over
+    # ss_extrema_u8 (one call per image, one stream)
+    d = ss.descs(imgs, outs)
     ws = ss.new_workspace(max(ss.workspace_size(w, h, ptile, pthreads) for h, w in sizes), dev)
-    for img, o in zip(imgs, outs):
-        ss.extrema(img, ptile, pthreads, ws, o)
-    torch.cuda.synchronize()
-    plain_ms = []
-    for _ in range(reps):
-        a, b = ev(), ev()
-        a.record(stream)
-        for img, o in zip(imgs, outs):
-            ss.extrema(img, ptile, pthreads, ws, o)
-        b.record(stream)
-        torch.cuda.synchronize()
-        plain_ms.append(a.elapsed_time(b))
+    plain_ms = ss.per_image_loop(d, ws, ptile, pthreads, reps=reps)
     ref = [(o[0].clone(), o[1].clone()) for o in outs[::step]]
     del ws
     # one graph-replayed batched pipeline
-    d = ss.descs(imgs, outs)
     wsb = ss.new_workspace(ss.batch_workspace_size(d, tile, threads), dev)
     plan = ss.BatchPlan(d, wsb, tile, threads)
     for o in outs:
@@ -142,12 +132,53 @@ def batched_leg(n, dev, ss, ssgen, torch, reps=3):
         "pixels": px, "tile": tile, "threads": threads, "per_image_tile": ptile, "per_image_threads": pthreads,
         "batched_gpx": round(px / (b_ms * 1e-3) / 1e9, 3), "batched_ms": round(b_ms, 3),
         "per_image_launch_gpx": round(px / (p_ms * 1e-3) / 1e9, 3), "per_image_launch_ms": round(p_ms, 3),
+        "per_image_launch_timing": "C++ loop over ss_extrema_u8 (libss_loop.so), hipEvents around each pass, "
+                                   "median of %d passes after one untimed pass" % reps,
         "speedup": round(p_ms / b_ms, 3),
         "batched_equals_per_image": bool(same), "fragments": st["fragments"], "dirty_tiles": st["dirty_tiles"],
     }
 
 
-def serving_leg(n, dev, ss, ssgen, torch, cap_bytes=32 << 30):
+def link_bandwidth(torch, dev, chunk=32 << 20, n=48, nstreams=3, reps=3):
+    """The host link under the serving traffic pattern with no kernels: n
+    "requests" of `chunk` pixels, each one H2D copy of 1 B/px then two D2H
+    copies of 1 B/px (the two masks), issued round-robin on nstreams streams
+    from pinned buffers (what the server does around its kernels).  Also the
+    H2D-only and D2H-only rates of the same pattern.  Median of reps, GB/s and
+    the pattern's Gpixel/s (its link bound)."""
+    h_in = [torch.empty(chunk, dtype=torch.uint8, pin_memory=True) for _ in range(nstreams)]
+    h_out = [torch.empty(2 * chunk, dtype=torch.uint8, pin_memory=True) for _ in range(nstreams)]
+    d_in = [torch.empty(chunk, dtype=torch.uint8, device=dev) for _ in range(nstreams)]
+    d_out = [torch.empty(2 * chunk, dtype=torch.uint8, device=dev) for _ in range(nstreams)]
+    streams = [torch.cuda.Stream() for _ in range(nstreams)]
+
+    def timed(h2d, d2h):
+        ts = []
+        for _ in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for k in range(n):
+                j = k % nstreams
+                with torch.cuda.stream(streams[j]):
+                    if h2d:
+                        d_in[j].copy_(h_in[j], non_blocking=True)
+                    if d2h:
+                        h_out[j][:chunk].copy_(d_out[j][:chunk], non_blocking=True)
+                        h_out[j][chunk:].copy_(d_out[j][chunk:], non_blocking=True)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return percentile(ts[1:], 50)
+
+    t_h2d, t_d2h, t_both = timed(True, False), timed(False, True), timed(True, True)
+    del h_in, h_out, d_in, d_out
+    return {"pattern": "%d requests of %d MiB px, 1 B/px H2D + 2 x 1 B/px D2H, %d streams, pinned" %
+                       (n, chunk >> 20, nstreams),
+            "h2d_gbs": round(n * chunk / t_h2d / 1e9, 2), "d2h_gbs": round(2 * n * chunk / t_d2h / 1e9, 2),
+            "both_gbs": round(3 * n * chunk / t_both / 1e9, 2),
+            "bound_gpx": round(n * chunk / t_both / 1e9, 3)}
+
+
+def serving_leg(n, dev, ss, ssgen, torch, cap_bytes=32 << 30, plain_cfg=None):
     """Open-loop Poisson requests (side log-uniform 64..8192, aspect 0.5..2,
     the three generators), host buffers in pinned memory, HBM admission cap 32
     GB.  Plain per-request path (H2D, one launch, D2H, in order) versus the
@@ -179,7 +210,8 @@ def serving_leg(n, dev, ss, ssgen, torch, cap_bytes=32 << 30):
     # arrival process (H2D, pipeline, D2H for each request in turn)
     # warm-up of each server: a saturated pass over all requests (first
     # launches of every kernel variant, first copies) before anything is timed
-    plain = ss.Server(hbm_cap_bytes=cap_bytes, max_batch=1, streams=1)
+    ptile, pthreads = plain_cfg or (tile, threads)
+    plain = ss.Server(hbm_cap_bytes=cap_bytes, max_batch=1, streams=1, tile=ptile, threads=pthreads)
     run(plain, 1e9, 3)
     _, _, plain_sat = run(plain, 1e9, 5)                 # capacity: all offered at once
     qps = 0.8 * plain_sat * 1e9 / (px / n)               # 80 % of the plain path's capacity
@@ -197,6 +229,9 @@ def serving_leg(n, dev, ss, ssgen, torch, cap_bytes=32 << 30):
     _, _, srv_sat = run(server, 1e9, 5)
     st = server.stats()
     server.close()
+    # the link bound: every request moves 1 B/px host->device and 2 B/px back;
+    # the same copies without kernels give the rate the link allows
+    link = link_bandwidth(torch, dev)
     return {
         "workload": "%d requests, side log-uniform 64..8192, aspect 0.5..2, iid / sigma 3 q16 / EDT, "
                     "pinned host buffers (H2D + pipeline + D2H per request), HBM cap %d GB, open-loop "
@@ -209,6 +244,9 @@ def serving_leg(n, dev, ss, ssgen, torch, cap_bytes=32 << 30):
         "mean_batch_at_qps": round(sum(c.batch_size for c in comps) / n, 2),
         "launches": st["launches"], "peak_inflight_bytes": st["peak_inflight_bytes"],
         "server_equals_plain": bool(same),
+        "plain_tile": ptile, "plain_threads": pthreads,
+        "link_bytes_per_px": 3, "link": link,
+        "server_saturated_frac_of_link_bound": round(srv_sat / link["bound_gpx"], 3),
     }
 
 
@@ -371,9 +409,8 @@ def main():
                                                         "seam_edge_fraction", "seam_fragments_per_mpix")}
     out["seam_sparsity"] = sparsity
 
-    # CPU-baseline sample (iid and sigma 4 / 16 levels when present), kept on the host
-    pick = [i for i, (n_, _) in enumerate(images) if n_ in ("iid", "smooth_s4_q16")] or [0]
-    cpu_sample = [(images[i][0], images[i][1].cpu()) for i in pick]
+    # CPU-baseline sample: every image of the step, kept on the host
+    cpu_sample = [(name, img.cpu().numpy()) for name, img in images] if not args.no_cpu else []
     del images, mx, mn, ws
     torch.cuda.empty_cache()
     if args.batch_images > 0:
@@ -383,20 +420,26 @@ def main():
         torch.cuda.empty_cache()
     if args.serve_requests > 0:
         t0 = time.time()
-        out["serving"] = serving_leg(args.serve_requests, dev, ss, ssgen, torch)
+        pcfg = None
+        if "batched" in out:
+            pcfg = (out["batched"]["per_image_tile"], out["batched"]["per_image_threads"])
+        out["serving"] = serving_leg(args.serve_requests, dev, ss, ssgen, torch, plain_cfg=pcfg)
         log("serving leg %.1fs" % (time.time() - t0), out["serving"])
         torch.cuda.empty_cache()
 
     if not args.no_cpu:
-        cores = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cores = args.cpu_threads or len(os.sched_getaffinity(0))
         t0 = time.time()
         rate, names = cpu_baseline(cpu_sample, cores)
-        rate1, _ = cpu_baseline(cpu_sample, 1, runs=1)
+        single = [s_ for s_ in cpu_sample if s_[0] in ("iid", "smooth_s4_q16")] or cpu_sample[:1]
+        rate1, names1 = cpu_baseline(single, 1, runs=1)
         out["cpu_baseline"] = {
             "value": round(rate, 4), "unit": "Gpixel/s", "cores": cores, "kind": "port",
-            "sample": "Global-UF oracle route on %s at %dx%d (median of 2 runs each after a "
-                      "warm-up, sum(px)/sum(median))" % ("+".join(names), side, side),
+            "sample": "Global-UF oracle route on all %d images of the step (%s) at %dx%d, median of 3 "
+                      "runs each after one warm-up run, sum(px)/sum(median)" % (len(names), ",".join(names),
+                                                                               side, side),
             "single_thread_value": round(rate1, 4),
+            "single_thread_sample": "%s, one run each" % ",".join(names1),
             "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
         }
         log("cpu baseline %.1fs" % (time.time() - t0))

@@ -388,6 +388,32 @@ class Server:
             pass
 
 
+_LOOP = None
+
+
+def per_image_loop(desc_array, ws, tile=None, threads=None, reps=3, stream=None):
+    """Times the plain path image by image from C++ (libss_loop.so: one
+    ss_extrema_u8 call per image, back to back on one stream, the reference's
+    per-image harness loop): one untimed pass, then `reps` timed passes.
+    Returns the per-pass device times in ms."""
+    global _LOOP
+    if _LOOP is None:
+        path = os.path.join(os.path.dirname(LIB_PATH), "libss_loop.so")
+        if not os.path.exists(path):
+            raise RuntimeError("libss_loop.so is not built (run __graft_entry__.build())")
+        _lib()   # the loop library links the operator library
+        lp = ctypes.CDLL(path)
+        lp.ss_loop_per_image.argtypes = [ctypes.POINTER(ImageDesc), ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+                                         ctypes.POINTER(Config), ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_float)]
+        lp.ss_loop_per_image.restype = ctypes.c_int
+        _LOOP = lp
+    out = (ctypes.c_float * reps)()
+    _check(_LOOP.ss_loop_per_image(desc_array, len(desc_array), ws.data_ptr(), ws.numel(), _cfg(tile, threads),
+                                   _stream_handle(stream), reps, out), "ss_loop_per_image")
+    return list(out)
+
+
 def profile_events(begin=None, end=None):
     """Route the tile-stage kernel through a pair of torch.cuda.Event objects
     (recorded on the pipeline's own stream); None, None turns it off."""
