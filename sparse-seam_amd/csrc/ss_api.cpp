@@ -53,9 +53,8 @@ ss_status extrema_single(const uint8_t* d_img, size_t pitch, int width, int heig
     const int64_t nt = tiles_of(width, height, c.tile);
     if (nt * (4 * c.tile - 4) > (int64_t)0xFFFFFFF0u) return SS_ERR_ARG;
     Images ims{};
-    ims.one = make_imgdev(d_img, d_max, d_min, (int64_t)pitch, (int64_t)out_pitch, width, height, c.tile, 0, 0, 0,
-                          0);
-    const Layout L = make_layout(nt, blocks_of(ims.one), seams_of(ims.one), regions_of(ims.one), c.tile, 1);
+    ims.one = make_imgdev(d_img, d_max, d_min, (int64_t)pitch, (int64_t)out_pitch, width, height, c.tile, 0, 0, 0);
+    const Layout L = make_layout(nt, blocks_of(ims.one), seams_of(ims.one), c.tile, 1);
     if (need) *need = L.bytes;
     if (!d_ws || ws_bytes < L.bytes || !aligned256(d_ws)) return SS_ERR_WORKSPACE;
     ims.table = nullptr;
@@ -115,8 +114,8 @@ ss_status ss_workspace_size(int width, int height, const ss_config* cfg, size_t*
     if (!config_valid(c.tile, c.threads)) return SS_ERR_ARG;
     const int64_t nt = tiles_of(width, height, c.tile);
     if (nt * (4 * c.tile - 4) > (int64_t)0xFFFFFFF0u) return SS_ERR_ARG;
-    const ImgDev d = make_imgdev(nullptr, nullptr, nullptr, width, width, width, height, c.tile, 0, 0, 0, 0);
-    *bytes = make_layout(nt, blocks_of(d), seams_of(d), regions_of(d), c.tile, 1).bytes;
+    const ImgDev d = make_imgdev(nullptr, nullptr, nullptr, width, width, width, height, c.tile, 0, 0, 0);
+    *bytes = make_layout(nt, blocks_of(d), seams_of(d), c.tile, 1).bytes;
     return SS_OK;
 }
 
@@ -163,9 +162,9 @@ ss_status ss_seam_profile_u8(const uint8_t* d_img, size_t pitch, int width, int 
     const int64_t nt = tiles_of(width, height, c.tile);
     if (nt * (4 * c.tile - 4) > (int64_t)0xFFFFFFF0u) return SS_ERR_ARG;
     Images ims{};
-    ims.one = make_imgdev(d_img, nullptr, nullptr, (int64_t)pitch, (int64_t)width, width, height, c.tile, 0, 0, 0, 0);
+    ims.one = make_imgdev(d_img, nullptr, nullptr, (int64_t)pitch, (int64_t)width, width, height, c.tile, 0, 0, 0);
     ims.count = 1;
-    const Layout L = make_layout(nt, blocks_of(ims.one), seams_of(ims.one), regions_of(ims.one), c.tile, 1);
+    const Layout L = make_layout(nt, blocks_of(ims.one), seams_of(ims.one), c.tile, 1);
     if (!d_ws || ws_bytes < L.bytes || !aligned256(d_ws)) return SS_ERR_WORKSPACE;
     const Ws ws = make_ws(d_ws, L, c.threads);
     if (launch_profile(ims, ws, c.tile, stream) != hipSuccess) return SS_ERR_HIP;
@@ -186,21 +185,20 @@ static ss_status batch_prepare(const ss_image_desc* images, int n, const ss_conf
     if (!images || n <= 0) return SS_ERR_ARG;
     if (!config_valid(c.tile, c.threads)) return SS_ERR_ARG;
     table.resize((size_t)n);
-    int64_t tiles = 0, blocks = 0, seams = 0, regions = 0;
+    int64_t tiles = 0, blocks = 0, seams = 0;
     for (int i = 0; i < n; ++i) {
         const ss_image_desc& d = images[i];
         if (d.width <= 0 || d.height <= 0 || d.pitch < (uint64_t)d.width || d.out_pitch < (uint64_t)d.width)
             return SS_ERR_ARG;
         ImgDev& t = table[(size_t)i];
         t = make_imgdev(d.img, d.max, d.min, (int64_t)d.pitch, (int64_t)d.out_pitch, d.width, d.height, c.tile,
-                        tiles, blocks, seams, regions);
+                        tiles, blocks, seams);
         tiles += (int64_t)t.tiles_x * t.tiles_y;
         blocks += blocks_of(t);
         seams += seams_of(t);
-        regions += regions_of(t);
     }
     if (tiles * (4 * c.tile - 4) > (int64_t)0xFFFFFFF0u) return SS_ERR_CAPACITY;
-    L = make_layout(tiles, blocks, seams, regions, c.tile, n);
+    L = make_layout(tiles, blocks, seams, c.tile, n);
     return SS_OK;
 }
 

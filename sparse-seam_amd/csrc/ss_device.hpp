@@ -68,10 +68,6 @@ struct ImgDev {
     // row but the last (seam_rows x tiles_x)
     int64_t seam_begin;
     int32_t seam_cols, seam_rows;
-    // T = 64 tile stage: regions of two horizontally adjacent tiles
-    // (regions_x per tile row), region_begin the global id of the first
-    int64_t region_begin;
-    int32_t regions_x, pad_;
 };
 
 struct Images {
@@ -127,9 +123,7 @@ struct Ws {
     uint32_t* tile_img;  // [tiles] image index of each tile (batches only)
     uint32_t* block_img; // [nblk] image index of each merge block (batches only)
     uint32_t* seam_img;  // [nseam] image index of each seam-merge unit (batches only)
-    uint32_t* region_img; // [nreg] image index of each two-tile region (batches only)
     int64_t ntiles;
-    int64_t nreg;        // two-tile regions (the T = 64 tile stage's workgroups)
     int64_t nblk;        // merge blocks
     int64_t nseam;       // seam-merge units (one tile edge each)
     uint32_t nblocks;    // scan blocks = ceil(ntiles / SCAN_CHUNK)

@@ -14,7 +14,6 @@ namespace ssx {
 // Byte offsets of the workspace regions (all 256-byte aligned).
 struct Layout {
     int64_t ntiles = 0;
-    int64_t nreg = 0;
     int64_t nblk = 0;
     int64_t nseam = 0;
     int tile = 0;
@@ -22,16 +21,15 @@ struct Layout {
     uint32_t nblocks = 0;
     size_t off_counts = 0, off_base = 0, off_bsum = 0, off_berr = 0, off_sbits = 0,
            off_rec8 = 0, off_rec = 0, off_eq = 0, off_dparent = 0, off_dbits = 0,
-           off_descs = 0, off_tile_img = 0, off_block_img = 0, off_seam_img = 0, off_region_img = 0;
+           off_descs = 0, off_tile_img = 0, off_block_img = 0, off_seam_img = 0;
     size_t bytes = 0;
 };
 
 inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
-inline Layout make_layout(int64_t ntiles, int64_t nblk, int64_t nseam, int64_t nreg, int T, int ndesc) {
+inline Layout make_layout(int64_t ntiles, int64_t nblk, int64_t nseam, int T, int ndesc) {
     Layout L;
     L.ntiles = ntiles;
-    L.nreg = nreg;
     L.nblk = nblk;
     L.nseam = nseam;
     L.tile = T;
@@ -58,7 +56,6 @@ inline Layout make_layout(int64_t ntiles, int64_t nblk, int64_t nseam, int64_t n
     L.off_tile_img = take(ndesc > 1 ? nt * 4 : 0);
     L.off_block_img = take(ndesc > 1 ? (size_t)nblk * 4 : 0);
     L.off_seam_img = take(ndesc > 1 ? (size_t)nseam * 4 : 0);
-    L.off_region_img = take(ndesc > 1 ? (size_t)nreg * 4 : 0);
     L.bytes = o;
     return L;
 }
@@ -80,9 +77,7 @@ inline Ws make_ws(void* base, const Layout& L, int threads) {
     w.tile_img = reinterpret_cast<uint32_t*>(b + L.off_tile_img);
     w.block_img = reinterpret_cast<uint32_t*>(b + L.off_block_img);
     w.seam_img = reinterpret_cast<uint32_t*>(b + L.off_seam_img);
-    w.region_img = reinterpret_cast<uint32_t*>(b + L.off_region_img);
     w.ntiles = L.ntiles;
-    w.nreg = L.nreg;
     w.nblk = L.nblk;
     w.nseam = L.nseam;
     w.nblocks = L.nblocks;
@@ -104,8 +99,7 @@ inline int64_t tiles_of(int w, int h, int T) {
 
 // Descriptor of one image: tile grid, merge-block grid and seam-merge units.
 inline ImgDev make_imgdev(const uint8_t* img, uint8_t* mx, uint8_t* mn, int64_t pitch, int64_t opitch, int w,
-                          int h, int T, int64_t tile_begin, int64_t block_begin, int64_t seam_begin,
-                          int64_t region_begin) {
+                          int h, int T, int64_t tile_begin, int64_t block_begin, int64_t seam_begin) {
     ImgDev d{};
     d.img = img;
     d.mx = mx;
@@ -124,11 +118,8 @@ inline ImgDev make_imgdev(const uint8_t* img, uint8_t* mx, uint8_t* mn, int64_t 
     // tile columns MERGE_BT k + MERGE_BT - 1 that have a column right of them
     d.seam_cols = (d.tiles_x - 1) / MERGE_BT;
     d.seam_rows = (d.tiles_y - 1) / MERGE_BT;
-    d.region_begin = region_begin;
-    d.regions_x = (d.tiles_x + 1) / 2;
     return d;
 }
-inline int64_t regions_of(const ImgDev& d) { return (int64_t)d.regions_x * d.tiles_y; }
 inline int64_t blocks_of(const ImgDev& d) { return (int64_t)d.blocks_x * d.blocks_y; }
 inline int64_t seams_of(const ImgDev& d) {
     return (int64_t)d.seam_cols * d.tiles_y + (int64_t)d.seam_rows * d.tiles_x;

@@ -14,7 +14,6 @@
 #include "ss_device.hpp"
 #include "ss_internal.hpp"
 #include "ss_tile.hpp"
-#include "ss_tile64.hpp"
 
 namespace ssx {
 
@@ -28,18 +27,8 @@ __global__ __launch_bounds__(NT) void k_tile_stage(Images ims, Ws ws) {
     tile_stage<T, NT>(sm, im, gtile, ws);
 }
 
-// T = 64: one workgroup per region of two horizontally adjacent tiles
-// (ss_tile64.hpp), XCD-aware region order.
-template <int WT, bool MULTI>
-__global__ __launch_bounds__(128 * WT) void k_tile_stage64(Images ims, Ws ws) {
-    __shared__ t64::Smem sm;
-    const int64_t reg = xcd_block(blockIdx.x, gridDim.x);
-    if (reg >= ws.nreg) return;
-    t64::region_stage<WT, MULTI>(sm, ims, ws, reg);
-}
-
-// Per-tile, per-merge-block, per-seam-unit and per-region image index of a
-// batch: one workgroup per image.
+// Per-tile, per-merge-block and per-seam-unit image index of a batch: one
+// workgroup per image.
 __global__ __launch_bounds__(256) void k_tile_map(Images ims, Ws ws) {
     const ImgDev im = ims.table[blockIdx.x];
     const int64_t n = (int64_t)im.tiles_x * im.tiles_y;
@@ -48,8 +37,6 @@ __global__ __launch_bounds__(256) void k_tile_map(Images ims, Ws ws) {
     for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) ws.block_img[im.block_begin + b] = blockIdx.x;
     const int64_t ns = (int64_t)im.seam_cols * im.tiles_y + (int64_t)im.seam_rows * im.tiles_x;
     for (int64_t u = threadIdx.x; u < ns; u += blockDim.x) ws.seam_img[im.seam_begin + u] = blockIdx.x;
-    const int64_t nr = (int64_t)im.regions_x * im.tiles_y;
-    for (int64_t r = threadIdx.x; r < nr; r += blockDim.x) ws.region_img[im.region_begin + r] = blockIdx.x;
 }
 
 // ---- compaction: exclusive scan of the per-tile counts --------------------
@@ -669,11 +656,7 @@ hipError_t launch_digest(const uint8_t* p, int64_t pitch, int w, int h, uint64_t
 // draining per-XCD queues needed 148 VGPRs per lane and ran slower).
 template <int T, int NT, bool MULTI>
 static void launch_tile_kernels(const Images& ims, const Ws& ws, hipStream_t s) {
-    if constexpr (T == 64) {
-        k_tile_stage64<NT / 64, MULTI><<<(unsigned)ws.nreg, 2 * NT, 0, s>>>(ims, ws);
-    } else {
-        k_tile_stage<T, NT, MULTI><<<(unsigned)ws.ntiles, NT, 0, s>>>(ims, ws);
-    }
+    k_tile_stage<T, NT, MULTI><<<(unsigned)ws.ntiles, NT, 0, s>>>(ims, ws);
 }
 
 template <int T, bool MULTI>
