@@ -129,13 +129,15 @@ ss_status ss_global_uf_u8(const uint8_t* d_img, size_t pitch, int width, int hei
 ss_status ss_read_tile_counts(const void* d_ws, size_t ws_bytes, uint32_t* out, size_t count,
                               hipStream_t stream);
 
-/* Seam-sparsity counters of one image at tile size cfg->tile (threads are
- * fixed at 256; NULL = the process default tile), the quantities the
+/* Seam-sparsity counters of one image at tile size cfg->tile (16, 32, 64 or
+ * 128 -- the reference profiler's default tiles are 32, 64 and 128; threads
+ * are fixed at 256; NULL = the process default tile), the quantities the
  * reference's seam profiler reports per image
  * (harness/seam_sparsity_profiler/gpu_seam_sparsity_profiler_v0_1.cu:133-294,
  * CSV columns :391-396; fractions and per-megapixel rates are ratios of these).
  * A diagnostic, not part of the extrema path: it stages and labels every tile
- * but writes no masks.  Uses the workspace of ss_workspace_size; synchronous. */
+ * but writes no masks.  Needs only a 256-byte aligned workspace of at least 256
+ * bytes (its counters; any ss_workspace_size workspace will do); synchronous. */
 typedef struct ss_seam_profile {
     uint64_t fragments;       /* tile-local zones, singletons included */
     uint64_t seam_fragments;  /* zones with an equal 8-neighbour in another tile */

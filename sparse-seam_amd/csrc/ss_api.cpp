@@ -158,14 +158,16 @@ ss_status ss_seam_profile_u8(const uint8_t* d_img, size_t pitch, int width, int 
     if (!d_img || !out || width <= 0 || height <= 0 || pitch < (size_t)width) return SS_ERR_ARG;
     ss_config c = resolve_config(cfg);
     c.threads = 256;
-    if (!config_valid(c.tile, c.threads)) return SS_ERR_ARG;
+    // the profiler also takes T = 128, the reference profiler's largest tile
+    if (c.tile != 128 && !config_valid(c.tile, c.threads)) return SS_ERR_ARG;
     const int64_t nt = tiles_of(width, height, c.tile);
     if (nt * (4 * c.tile - 4) > (int64_t)0xFFFFFFF0u) return SS_ERR_ARG;
     Images ims{};
     ims.one = make_imgdev(d_img, nullptr, nullptr, (int64_t)pitch, (int64_t)width, width, height, c.tile, 0, 0, 0);
     ims.count = 1;
     const Layout L = make_layout(nt, blocks_of(ims.one), seams_of(ims.one), c.tile, 1);
-    if (!d_ws || ws_bytes < L.bytes || !aligned256(d_ws)) return SS_ERR_WORKSPACE;
+    // the counters live in the workspace header; nothing else is used
+    if (!d_ws || ws_bytes < sizeof(WsHeader) || !aligned256(d_ws)) return SS_ERR_WORKSPACE;
     const Ws ws = make_ws(d_ws, L, c.threads);
     if (launch_profile(ims, ws, c.tile, stream) != hipSuccess) return SS_ERR_HIP;
     uint64_t v[5];

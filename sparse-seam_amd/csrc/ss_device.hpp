@@ -40,7 +40,7 @@ namespace ssx {
 constexpr uint32_t F_G = 1u;      // a strictly greater 8-neighbour exists
 constexpr uint32_t F_S = 2u;      // a strictly smaller 8-neighbour exists
 constexpr uint32_t F_SEAM = 4u;   // an equal 8-neighbour lies in another tile
-constexpr uint32_t LMASK = 0xFFFu;      // label bits (T*T <= 4096)
+constexpr uint32_t LMASK = 0x3FFFu;     // label bits (T*T <= 16384: the profiler also runs T = 128)
 constexpr int FLAG_SHIFT = 16;          // zone evidence OR'ed at the root
 constexpr int FRAG_SHIFT = 20;          // fragment id + 1 stored at seam roots
 constexpr uint32_t DB_BROOT = 16u;      // dense bits: root of a block-local component

@@ -600,6 +600,7 @@ hipError_t launch_profile(const Images& ims, const Ws& ws, int T, hipStream_t s)
     case 16: k_seam_profile<16><<<grid, 256, 0, s>>>(ims, ws); break;
     case 32: k_seam_profile<32><<<grid, 256, 0, s>>>(ims, ws); break;
     case 64: k_seam_profile<64><<<grid, 256, 0, s>>>(ims, ws); break;
+    case 128: k_seam_profile<128><<<grid, 256, 0, s>>>(ims, ws); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -22,14 +22,14 @@
 //
 // Writes every input as the 8-bit gray PGM the other modes read (no GPU).
 //
-//   ss_harness --seam-profile <image_dir> --output=FILE [--tiles=16,32,64]
+//   ss_harness --seam-profile <image_dir> --output=FILE [--tiles=32,64,128]
 //              [--levels=256,32,8,2] [--device=D]
 //
 // Seam-sparsity mode, mirroring the reference's profiler
 // (harness/seam_sparsity_profiler/gpu_seam_sparsity_profiler_v0_1.cu: CLI
 // :341-383, per image x levels x tiles rows :391-517, exit codes :518-540):
 // one CSV row per (image, levels, tile) with the same columns.  Tiles are
-// limited to the operator's 16, 32 and 64.  The reference quantises to bin
+// 16, 32, 64 or 128 (default 32, 64, 128 as in the reference).  The reference quantises to bin
 // indices ((v L) >> 8) there; the rescaled quantiser used here maps the same
 // bins to distinct values, so every count is the same.
 #include <hip/hip_runtime.h>
@@ -279,14 +279,14 @@ int convert_main(int argc, char** argv) {
 
 int seam_profile_main(int argc, char** argv) {
     if (argc < 3) {
-        std::cerr << "Usage: ss_harness --seam-profile <image_dir> --output=FILE [--tiles=16,32,64]"
+        std::cerr << "Usage: ss_harness --seam-profile <image_dir> --output=FILE [--tiles=32,64,128]"
                      " [--levels=256,32,8,2] [--device=D]\n";
         return 2;
     }
     try {
         const fs::path image_dir = argv[2];
         fs::path output;
-        std::vector<int> tiles{16, 32, 64}, levels{256, 32, 8, 2};
+        std::vector<int> tiles{32, 64, 128}, levels{256, 32, 8, 2};
         int device = 0;
         for (int i = 3; i < argc; ++i) {
             const std::string a = argv[i];
@@ -301,7 +301,8 @@ int seam_profile_main(int argc, char** argv) {
         }
         if (output.empty()) throw std::runtime_error("Missing --output file");
         for (int t : tiles)
-            if (t != 16 && t != 32 && t != 64) throw std::runtime_error("Tile size must be 16, 32 or 64");
+            if (t != 16 && t != 32 && t != 64 && t != 128)
+                throw std::runtime_error("Tile size must be 16, 32, 64 or 128");
         for (int l : levels)
             if (l < 2 || l > 256) throw std::runtime_error("Levels must be in [2,256]");
         SS_HIP_CHECK(hipSetDevice(device));
@@ -321,13 +322,7 @@ int seam_profile_main(int argc, char** argv) {
             const double mp = (double)px / 1e6;
             uint8_t* d_img = nullptr;
             void* d_ws = nullptr;
-            size_t ws_bytes = 0;
-            for (int t : tiles) {
-                size_t need = 0;
-                const ss_config c{t, 256};
-                sparse_seam::check(ss_workspace_size(src.width, src.height, &c, &need), "ss_workspace_size");
-                ws_bytes = std::max(ws_bytes, need);
-            }
+            const size_t ws_bytes = 256;   // the profiler's counters only
             SS_HIP_CHECK(hipMalloc(&d_img, px));
             SS_HIP_CHECK(hipMalloc(&d_ws, ws_bytes));
             for (int level : levels) {

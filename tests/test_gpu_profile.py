@@ -33,7 +33,7 @@ def test_profile_random(seed):
     img = ssgen.quantize(torch.as_tensor(rng.integers(0, 256, (h, w), dtype=np.uint8)), levels).numpy()
     if seed % 2:
         img = np.repeat(np.repeat(img[: -(-h // 3), : -(-w // 3)], 3, 0), 3, 1)[:h, :w].copy()
-    for tile in (16, 32, 64):
+    for tile in (16, 32, 64, 128):
         say("profile random", seed, (h, w), levels, tile)
         assert profile(img, tile) == oracle_seam_profile(img, tile), (seed, tile)
 
@@ -41,7 +41,7 @@ def test_profile_random(seed):
 def test_profile_golden_sample():
     cases = list(golden_cases())[::17]
     for suite, name, img, _, _, _ in cases:
-        for tile in (16, 64):
+        for tile in (16, 64, 128):
             say("profile golden", name, tile)
             assert profile(img, tile) == oracle_seam_profile(img, tile), (name, tile)
 

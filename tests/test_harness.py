@@ -30,7 +30,7 @@ def test_usage_and_argument_errors():
     assert run(["--seam-profile", GOLDEN, "--bogus"]).returncode == 2
     r = run(["--seam-profile", GOLDEN])           # no --output
     assert r.returncode == 3 and "Missing --output" in r.stderr
-    r = run(["--seam-profile", GOLDEN, "--output=/tmp/x.csv", "--tiles=128"])
+    r = run(["--seam-profile", GOLDEN, "--output=/tmp/x.csv", "--tiles=96"])
     assert r.returncode == 3 and "Tile size" in r.stderr
 
 
@@ -85,7 +85,7 @@ def test_seam_profile_mode(tmp_path):
     the oracle restatement on the quantised inputs."""
     root = os.path.join(GOLDEN, "oracle_v0_3_sparse_seam_adversarial", "inputs")
     out = tmp_path / "profile.csv"
-    r = run(["--seam-profile", root, "--output=%s" % out, "--tiles=16,64", "--levels=256,4"])
+    r = run(["--seam-profile", root, "--output=%s" % out, "--tiles=16,128", "--levels=256,4"])
     assert r.returncode == 0, r.stderr[-2000:]
     assert "[SS-SEAM-PROFILE-OK]" in r.stdout
     rows = list(csv.DictReader(open(out)))
