@@ -177,10 +177,11 @@ constexpr uint32_t B_G = 0x80808080u;     // a strictly greater 8-neighbour exis
 constexpr uint32_t B_S = 0x40404040u;     // a strictly smaller 8-neighbour exists
 constexpr uint32_t B_SGL = 0x20202020u;   // singleton
 constexpr uint32_t B_VAL = 0x10101010u;   // pixel inside the image part of the tile
-constexpr uint32_t B_RT = 0x01010101u;    // run head whose Komura link is itself
-// The four B_RT bits of a byte-form word as a 4-bit mask: the only heads that
-// can end as roots (a union only ever turns a root into a non-root).
-__device__ __forceinline__ uint32_t b_roots(uint32_t sb) { return (((sb & B_RT) * 0x01020408u) >> 24) & 0xFu; }
+// Bits 8-11 (free in the byte form, set by store_labels): the run heads whose
+// Komura link is the head itself, the only heads that can end as roots (a
+// union only ever turns a root into a non-root).
+constexpr int B_RT_SHIFT = 8;
+__device__ __forceinline__ uint32_t b_roots(uint32_t sb) { return (sb >> B_RT_SHIFT) & 0xFu; }
 // 0x80-per-byte flags -> 0xFF bytes.
 __device__ __forceinline__ uint32_t bytes80(uint32_t x) { return (x - (x >> 7)) | x; }
 // Two 0x80-per-byte flag words -> one byte: a's pixels in bits 0-3, b's in 4-7.
@@ -388,7 +389,6 @@ __device__ __forceinline__ uint32_t classify_quad(const RowW& a, const RowW& b, 
     const uint32_t sgl80 = neq & val80;
     all_eq = all_eq && (ne_top(V, v4) & val80) == 0;
     st = 0;
-    uint32_t rtb = 0;
     if (val80 & ~neq) {
         const uint32_t lv80 = lx0 > 0 ? 0x80808080u : 0x80808000u;   // left neighbour in the tile
         const uint32_t rv80 = first_bytes80(tw - lx0 - 1);              // right neighbour in the tile
@@ -398,16 +398,11 @@ __device__ __forceinline__ uint32_t classify_quad(const RowW& a, const RowW& b, 
         uint32_t s = movemask_pair(g80, s80) << Q_G | movemask_pair(el80, eu80) << Q_EL |
                      movemask_pair(eul80, eur80) << Q_EUL;
         const uint32_t nsg4 = movemask_pair(val80 & ~neq, 0u);
-        s |= (nsg4 & ~((s >> Q_EL) & 0xEu)) << Q_HD;
-        // Union and root-candidate heads of the quad; its labels are stored
-        // by store_labels once the staged rows are no longer read.
-        const uint32_t p0 = ly * T + lx0;
-        uint32_t lab4[4], un, rt;
-        quad_links(s, p0, (uint32_t)T, lab4, un, rt);
-        st = s | un << Q_UN;
-        rtb = (rt * 0x00204081u) & B_RT;
+        // the links, union heads and root candidates follow from s alone:
+        // store_labels derives them once the staged rows are no longer read
+        st = s | (nsg4 & ~((s >> Q_EL) & 0xEu)) << Q_HD;
     }
-    return g80 | (s80 >> 1) | (sgl80 >> 2) | (val80 >> 3) | rtb;
+    return g80 | (s80 >> 1) | (sgl80 >> 2) | (val80 >> 3);
 }
 
 // First pass over the thread's quads (a vertical strip): evidence, singleton
@@ -447,14 +442,17 @@ __device__ __forceinline__ bool quad_init(TileSmem<T>& sm, int tw, int th, uint3
     return all_eq;
 }
 
-// Komura labels of the thread's non-singleton quads, recomputed from their
-// edge words, one 16-byte store per quad: a head links to its first contact
-// above (else to the pixel left of the quad, else to itself), every other run
-// pixel to its head; a singleton in such a quad gets a dummy entry that nothing
-// reads.  Every link points at a smaller index.  Called once nothing reads the
-// staged rows any more (the labels overwrite them) and followed by a barrier.
+// Komura labels of the thread's non-singleton quads, derived from their edge
+// words, one 16-byte store per quad: a head links to its first contact above
+// (else to the pixel left of the quad, else to itself), every other run pixel
+// to its head; a singleton in such a quad gets a dummy entry that nothing
+// reads.  Every link points at a smaller index.  Also completes the quad
+// words: the union heads into st (Q_UN), the root candidates into sb (bits
+// B_RT_SHIFT ..).  Called once nothing reads the staged rows any more (the
+// labels overwrite them) and followed by a barrier.
 template <int T, int NT>
-__device__ __forceinline__ void store_labels(TileSmem<T>& sm, const uint32_t (&st)[Quads<T, NT>::PER]) {
+__device__ __forceinline__ void store_labels(TileSmem<T>& sm, uint32_t (&st)[Quads<T, NT>::PER],
+                                             uint32_t (&sb)[Quads<T, NT>::PER]) {
 #pragma unroll
     for (int k = 0; k < Quads<T, NT>::PER; ++k) {
         if (!st[k]) continue;
@@ -464,6 +462,8 @@ __device__ __forceinline__ void store_labels(TileSmem<T>& sm, const uint32_t (&s
         uint32_t lab4[4], un, rt;
         quad_links(st[k], p0, (uint32_t)T, lab4, un, rt);
         *reinterpret_cast<uint4*>(&sm.lab[p0]) = make_uint4(lab4[0], lab4[1], lab4[2], lab4[3]);
+        st[k] |= un << Q_UN;
+        sb[k] |= rt << B_RT_SHIFT;
     }
 }
 
@@ -673,7 +673,7 @@ __device__ __forceinline__ void tile_stage(TileSmem<T>& sm, const ImgDev& im, in
         }
         write_perimeter_records<T, NT>(sm, rec, tw, th, seam_bits, true);
     } else {
-        store_labels<T, NT>(sm, st);
+        store_labels<T, NT>(sm, st, sb);
         __syncthreads();
         quad_label<T, NT>(sm, st, sb, tw, th, seam_bits);
         // Number the seam zones (their roots) and fill their fragment slots.
@@ -804,7 +804,7 @@ __device__ __forceinline__ void tile_profile(TileSmem<T>& sm, const ImgDev& im, 
         }
     } else {
         __syncthreads();   // the equal-pair counts above read the staged rows
-        store_labels<T, NT>(sm, st);
+        store_labels<T, NT>(sm, st, sb);
         __syncthreads();
         quad_label<T, NT>(sm, st, sb, tw, th, seam_bits);
 #pragma unroll
