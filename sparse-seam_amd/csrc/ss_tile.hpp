@@ -88,6 +88,13 @@ struct TileSmem {
     uint32_t err;
     uint32_t zflags;                  // uniform tile: OR of the evidence
     uint32_t open;                    // the tile holds a fragment that may still change
+    // Work list of the union pass (quad_label): the unions are spread over all
+    // threads of the workgroup, so the pass takes as many rounds as the tile
+    // has unions per thread rather than as many as its busiest thread has.
+    // Unions beyond the capacity are done at once by the thread that has them.
+    static constexpr int CAP_PAIRS = T * T / 8;
+    uint32_t pairs[CAP_PAIRS];        // pixel index | pixel index << 16
+    uint32_t npairs;
 };
 
 // Thread -> quad mapping.  With fewer threads than quads a thread owns a
@@ -528,12 +535,20 @@ __device__ __forceinline__ void quad_label(TileSmem<T>& sm, const uint32_t (&st)
                                            uint32_t seam_bits) {
     using QS = QuadSet<T, NT>;
     const uint32_t base = QS::base();
+    if (threadIdx.x == 0) sm.npairs = 0;
     // a self-linked head is already compressed
     QS::each(QS::gather([&](int k) { return q_heads(st[k]) & ~b_roots(sb[k]); }), [&](int k, int i) {
         const uint32_t p = base + k * T + i;
         sm.lab[p] = lfind_h(sm.lab, p);
     });
     __syncthreads();
+    auto unite = [&](uint32_t a, uint32_t b) {
+        const uint32_t j = atomicAdd(&sm.npairs, 1u);
+        if (j < (uint32_t)TileSmem<T>::CAP_PAIRS)
+            sm.pairs[j] = a | b << 16;
+        else
+            lunion_h(sm.lab, a, b);
+    };
     QS::each(QS::gather([&](int k) { return q_unions(st[k]); }), [&](int k, int i) {
         const uint32_t s = QS::pick(st, k), p0 = base + k * T;
         const uint32_t ab = q_above(s, q_run(s, i));
@@ -542,13 +557,21 @@ __device__ __forceinline__ void quad_label(TileSmem<T>& sm, const uint32_t (&st)
         while (seg) {
             const int j = __builtin_ctz(seg);
             seg &= seg - 1u;
-            lunion_h(sm.lab, p0 + i, p0 - T + j - 1);
+            unite(p0 + i, p0 - T + j - 1);
         }
         // The left pixel needs a union only if the run touches neither of
         // its two upper neighbours: such a contact is one of the left
         // pixel's own contacts, to which both runs are already joined.
-        if (i == 0 && ((s >> Q_EL) & 1u) && !(ab & 3u)) lunion_h(sm.lab, p0, p0 - 1);
+        if (i == 0 && ((s >> Q_EL) & 1u) && !(ab & 3u)) unite(p0, p0 - 1);
     });
+    __syncthreads();
+    {
+        const uint32_t n = min(sm.npairs, (uint32_t)TileSmem<T>::CAP_PAIRS);
+        for (uint32_t j = threadIdx.x; j < n; j += NT) {
+            const uint32_t w = sm.pairs[j];
+            lunion_h(sm.lab, w & 0xFFFFu, w >> 16);
+        }
+    }
     __syncthreads();
     QS::each(QS::gather([&](int k) { return q_heads(st[k]); }), [&](int k, int i) {
         const uint32_t s = QS::pick(st, k), p = base + k * T + i;
