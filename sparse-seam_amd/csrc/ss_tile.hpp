@@ -863,13 +863,15 @@ __device__ __forceinline__ void tile_profile(TileSmem<T>& sm, const ImgDev& im, 
 // cw is the tile's counts word; only a tile with an open fragment, or a
 // uniform tile (whose one fragment may be open), can change.  Lanes resolve
 // the tile's fragments against the seam union-find into the wave's LDS table
-// (bit 7: the fragment changed; bits 0-1: its final max / min masks).  If any
-// changed, the wave rewrites the tile: a uniform tile is one fragment; any
-// other tile streams its record plane (fragment ids, or the final masks of
-// closed pixels) and rewrites every whole tile row that holds a changed pixel,
-// so HBM sees full row segments rather than scattered partial lines.  No image
-// read, no relabelling.  b is the tile's base (dense id of its first
-// fragment).  Returns whether the tile changed.
+// (bits 6 / 7: the fragment's max / min mask changed; bits 0-1: its final max /
+// min masks).  If any changed, the wave rewrites the tile: a uniform tile is
+// one fragment; any other tile streams its record plane (fragment ids, or the
+// final masks of closed pixels) and rewrites, per mask, every whole tile row
+// that holds a pixel whose value in that mask changed, so HBM sees full row
+// segments rather than scattered partial lines, and a mask no fragment of the
+// row changed in is not written.  No image read, no relabelling.  b is the
+// tile's base (dense id of its first fragment).  Returns whether the tile
+// changed.
 template <int T>
 __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, int64_t gtile, uint32_t cw,
                                                uint32_t b, const Ws& ws) {
@@ -880,9 +882,10 @@ __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, i
         const uint32_t own = ws.dbits[b + j];
         uint32_t res = own & 3u;
         if (res != 3u) res |= (ws.dbits[gfind_ro(ws.dparent, b + j)] >> 2) & 3u;
-        const bool ch = (res & ~own & 3u) != 0;
-        changed = changed || ch;
-        lut[j] = (uint8_t)((ch ? 0x80u : 0u) | ((res & F_G) ? 0u : 1u) | ((res & F_S) ? 0u : 2u));
+        // gained greater evidence: the max mask changes; smaller: the min mask
+        const uint32_t gain = res & ~own & 3u;
+        changed = changed || gain != 0;
+        lut[j] = (uint8_t)(gain << 6 | ((res & F_G) ? 0u : 1u) | ((res & F_S) ? 0u : 2u));
     }
     if (!__ballot(changed)) return false;
     // the table was written by other lanes of this wave: LDS operations of one
@@ -897,13 +900,14 @@ __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, i
     if (cw & CNT_UNIFORM) {
         const uint32_t v = lut[0];
         const uint32_t mxb = (v & 1u) ? 0xFFFFFFFFu : 0u, mnb = (v & 2u) ? 0xFFFFFFFFu : 0u;
+        const bool wx = (v & 0x40u) != 0, wn = (v & 0x80u) != 0;
         for (int q = lane; q < T * T / 4; q += 64) {
             const int ly = q / (T / 4), lx0 = 4 * (q % (T / 4));
             if (ly >= th || lx0 >= tw) continue;
             const uint32_t sel = tw - lx0 >= 4 ? 0xFu : ((1u << (tw - lx0)) - 1u);
             const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + lx0;
-            store_quad(im.mx + o, mxb, sel);
-            store_quad(im.mn + o, mnb, sel);
+            if (wx) store_quad(im.mx + o, mxb, sel);
+            if (wn) store_quad(im.mn + o, mnb, sel);
         }
         return true;
     }
@@ -921,7 +925,7 @@ __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, i
         const uint4 rv = *reinterpret_cast<const uint4*>(plane + ly * T + c0);
         uint32_t rw[4] = {rv.x, rv.y, rv.z, rv.w};
         uint32_t mxw[4], mnw[4];
-        bool ch = false;
+        uint32_t ch = 0;   // bit 6: a max value changed, bit 7: a min value
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
             if (ly >= th) {
@@ -942,27 +946,28 @@ __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, i
                 for (int i = 0; i < 4; ++i) {
                     const uint32_t r = (w >> (8 * i)) & 0xFFu;
                     const uint32_t v = r >= REC_CLOSED ? (r & 3u) : (uint32_t)lut[r];
-                    ch = ch || (v & 0x80u) != 0;
+                    ch |= v;
                     mxw[d] |= ((v & 1u) ? 0xFFu : 0u) << (8 * i);
                     mnw[d] |= ((v & 2u) ? 0xFFu : 0u) << (8 * i);
                 }
             }
         }
-        const uint64_t bal = __ballot(ch);
-        if (!((bal >> (lane & ~(VR - 1))) & ((1u << VR) - 1u))) continue;
-        if (ly >= th || c0 >= tw) continue;
+        const uint32_t rowbits = (1u << VR) - 1u, first = (uint32_t)lane & ~(VR - 1u);
+        const bool wx = (__ballot((ch & 0x40u) != 0) >> first) & rowbits;
+        const bool wn = (__ballot((ch & 0x80u) != 0) >> first) & rowbits;
+        if (!(wx || wn) || ly >= th || c0 >= tw) continue;
         const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + c0;
         if (c0 + 16 <= tw) {
-            store_u128(im.mx + o, make_uint4(mxw[0], mxw[1], mxw[2], mxw[3]));
-            store_u128(im.mn + o, make_uint4(mnw[0], mnw[1], mnw[2], mnw[3]));
+            if (wx) store_u128(im.mx + o, make_uint4(mxw[0], mxw[1], mxw[2], mxw[3]));
+            if (wn) store_u128(im.mn + o, make_uint4(mnw[0], mnw[1], mnw[2], mnw[3]));
         } else {
 #pragma unroll
             for (int d = 0; d < 4; ++d) {
                 const int nv = tw - c0 - 4 * d;
                 if (nv <= 0) break;
                 const uint32_t sel = nv >= 4 ? 0xFu : ((1u << nv) - 1u);
-                store_quad(im.mx + o + 4 * d, mxw[d], sel);
-                store_quad(im.mn + o + 4 * d, mnw[d], sel);
+                if (wx) store_quad(im.mx + o + 4 * d, mxw[d], sel);
+                if (wn) store_quad(im.mn + o + 4 * d, mnw[d], sel);
             }
         }
     }
