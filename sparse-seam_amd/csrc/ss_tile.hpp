@@ -93,7 +93,13 @@ struct TileSmem {
     // has unions per thread rather than as many as its busiest thread has.
     // Unions beyond the capacity are done at once by the thread that has them.
     static constexpr int CAP_PAIRS = T * T / 8;
-    uint32_t pairs[CAP_PAIRS];        // pixel index | pixel index << 16
+    // Staging of store_labels (earlier in the tile, so the two share LDS):
+    // one slot per thread of a workgroup of up to STAGE threads.
+    static constexpr int STAGE = 256;
+    union {
+        uint32_t pairs[CAP_PAIRS];    // pixel index | pixel index << 16
+        uint2 stage[STAGE];           // quad word, pixel index of its pixel 0
+    };
     uint32_t npairs;
 };
 
@@ -457,20 +463,74 @@ __device__ __forceinline__ bool quad_init(TileSmem<T>& sm, int tw, int th, uint3
 // words: the union heads into st (Q_UN), the root candidates into sb (bits
 // B_RT_SHIFT ..).  Called once nothing reads the staged rows any more (the
 // labels overwrite them) and followed by a barrier.
+//
+// A wave whose non-singleton quads are few (noise-like content: most quads
+// hold singletons only) spreads them over its lanes: each is staged in LDS
+// by its owner, taken by the lane of its rank, and its union heads and root
+// candidates are handed back; otherwise every lane would step through all its
+// quads for the sake of the few lanes that have work in each.
 template <int T, int NT>
 __device__ __forceinline__ void store_labels(TileSmem<T>& sm, uint32_t (&st)[Quads<T, NT>::PER],
                                              uint32_t (&sb)[Quads<T, NT>::PER]) {
+    constexpr int PER = Quads<T, NT>::PER;
+    auto links = [&](uint32_t s, uint32_t p0) {
+        uint32_t lab4[4], un, rt;
+        quad_links(s, p0, (uint32_t)T, lab4, un, rt);
+        *reinterpret_cast<uint4*>(&sm.lab[p0]) = make_uint4(lab4[0], lab4[1], lab4[2], lab4[3]);
+        return un | rt << 4;
+    };
+    if constexpr (PER > 1 && NT <= TileSmem<T>::STAGE) {
+        uint64_t bal[PER];
+        uint32_t rank[PER], n = 0;
 #pragma unroll
-    for (int k = 0; k < Quads<T, NT>::PER; ++k) {
+        for (int k = 0; k < PER; ++k) {
+            bal[k] = __ballot(st[k] != 0);
+            rank[k] = n + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal[k] >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)bal[k], 0u));
+            n += (uint32_t)__builtin_popcountll(bal[k]);
+        }
+        if (n <= 2 * 64) {   // wave-uniform
+            uint2* stage = sm.stage + (threadIdx.x & ~63u);
+            const uint32_t lane = threadIdx.x & 63u;
+            auto wave_sync = [] {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            };
+            for (uint32_t r0 = 0; r0 < n; r0 += 64) {
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    if (!st[k] || rank[k] - r0 >= 64u) continue;
+                    int ly, lx0;
+                    Quads<T, NT>::at(k, ly, lx0);
+                    stage[rank[k] - r0] = make_uint2(st[k], (uint32_t)(ly * T + lx0));
+                }
+                wave_sync();
+                if (lane < n - r0) {
+                    const uint2 it = stage[lane];
+                    stage[lane].x = links(it.x, it.y);
+                }
+                wave_sync();
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    if (!st[k] || rank[k] - r0 >= 64u) continue;
+                    const uint32_t res = stage[rank[k] - r0].x;
+                    st[k] |= (res & 0xFu) << Q_UN;
+                    sb[k] |= (res >> 4) << B_RT_SHIFT;
+                }
+                wave_sync();
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
         if (!st[k]) continue;
         int ly, lx0;
         Quads<T, NT>::at(k, ly, lx0);
-        const uint32_t p0 = ly * T + lx0;
-        uint32_t lab4[4], un, rt;
-        quad_links(st[k], p0, (uint32_t)T, lab4, un, rt);
-        *reinterpret_cast<uint4*>(&sm.lab[p0]) = make_uint4(lab4[0], lab4[1], lab4[2], lab4[3]);
-        st[k] |= un << Q_UN;
-        sb[k] |= rt << B_RT_SHIFT;
+        const uint32_t res = links(st[k], (uint32_t)(ly * T + lx0));
+        st[k] |= (res & 0xFu) << Q_UN;
+        sb[k] |= (res >> 4) << B_RT_SHIFT;
     }
 }
 
