@@ -94,4 +94,19 @@ __host__ __device__ __forceinline__ void quad_links(uint32_t s, uint32_t p0, uin
     }
 }
 
+// quad_links for a quad that is one run (pixels 1..3 continue the run of
+// pixel 0, so pixel 0 is the only head): the run's contacts are the OR of the
+// three contact fields.  Same results as quad_links for such a quad.
+__host__ __device__ __forceinline__ bool q_one_run(uint32_t s) { return ((s >> Q_EL) & 0xEu) == 0xEu; }
+__host__ __device__ __forceinline__ void quad_links_one_run(uint32_t s, uint32_t p0, uint32_t T, uint32_t (&lab4)[4],
+                                                            uint32_t& un, uint32_t& rt) {
+    const uint32_t r0 = (((s >> Q_EUL) & 0xFu) | ((s >> Q_EU) & 0xFu) << 1 | ((s >> Q_EUR) & 0xFu) << 2);
+    const uint32_t left = (s >> Q_EL) & 1u;
+    const uint32_t sg = r0 & ~(r0 << 1);   // first column of each contact segment
+    lab4[0] = r0 ? p0 - T - 1u + __builtin_ctz(r0) : (left ? p0 - 1u : p0);
+    lab4[1] = lab4[2] = lab4[3] = p0;
+    un = ((sg & (sg - 1u)) != 0u || (left && r0 && !(r0 & 3u))) ? 1u : 0u;
+    rt = (!r0 && !left) ? 1u : 0u;
+}
+
 }  // namespace ssx

@@ -464,11 +464,13 @@ __device__ __forceinline__ bool quad_init(TileSmem<T>& sm, int tw, int th, uint3
 // B_RT_SHIFT ..).  Called once nothing reads the staged rows any more (the
 // labels overwrite them) and followed by a barrier.
 //
-// A wave whose non-singleton quads are few (noise-like content: most quads
-// hold singletons only) spreads them over its lanes: each is staged in LDS
-// by its owner, taken by the lane of its rank, and its union heads and root
-// candidates are handed back; otherwise every lane would step through all its
-// quads for the sake of the few lanes that have work in each.
+// A quad that is one run (a plateau's interior) takes the short form
+// quad_links_one_run.  A wave whose other non-singleton quads are few (noise-
+// like content: most quads hold singletons only; plateaus: only their rims)
+// spreads those over its lanes: each is staged in LDS by its owner, taken by
+// the lane of its rank, and its union heads and root candidates are handed
+// back; otherwise every lane would step through all its quads for the sake of
+// the few lanes that have work in each.
 template <int T, int NT>
 __device__ __forceinline__ void store_labels(TileSmem<T>& sm, uint32_t (&st)[Quads<T, NT>::PER],
                                              uint32_t (&sb)[Quads<T, NT>::PER]) {
@@ -482,14 +484,28 @@ __device__ __forceinline__ void store_labels(TileSmem<T>& sm, uint32_t (&st)[Qua
     if constexpr (PER > 1 && NT <= TileSmem<T>::STAGE) {
         uint64_t bal[PER];
         uint32_t rank[PER], n = 0;
+        bool gen[PER];   // needs the general form
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            bal[k] = __ballot(st[k] != 0);
+            gen[k] = st[k] != 0 && !q_one_run(st[k]);
+            bal[k] = __ballot(gen[k]);
             rank[k] = n + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal[k] >> 32),
                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)bal[k], 0u));
             n += (uint32_t)__builtin_popcountll(bal[k]);
         }
-        if (n <= 2 * 64) {   // wave-uniform
+        if (n <= 64) {   // wave-uniform: one round
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                if (!st[k] || gen[k]) continue;
+                int ly, lx0;
+                Quads<T, NT>::at(k, ly, lx0);
+                const uint32_t p0 = ly * T + lx0;
+                uint32_t lab4[4], un, rt;
+                quad_links_one_run(st[k], p0, (uint32_t)T, lab4, un, rt);
+                *reinterpret_cast<uint4*>(&sm.lab[p0]) = make_uint4(lab4[0], lab4[1], lab4[2], lab4[3]);
+                st[k] |= un << Q_UN;
+                sb[k] |= rt << B_RT_SHIFT;
+            }
             uint2* stage = sm.stage + (threadIdx.x & ~63u);
             const uint32_t lane = threadIdx.x & 63u;
             auto wave_sync = [] {
@@ -500,7 +516,7 @@ __device__ __forceinline__ void store_labels(TileSmem<T>& sm, uint32_t (&st)[Qua
             for (uint32_t r0 = 0; r0 < n; r0 += 64) {
 #pragma unroll
                 for (int k = 0; k < PER; ++k) {
-                    if (!st[k] || rank[k] - r0 >= 64u) continue;
+                    if (!gen[k] || rank[k] - r0 >= 64u) continue;
                     int ly, lx0;
                     Quads<T, NT>::at(k, ly, lx0);
                     stage[rank[k] - r0] = make_uint2(st[k], (uint32_t)(ly * T + lx0));
@@ -513,7 +529,7 @@ __device__ __forceinline__ void store_labels(TileSmem<T>& sm, uint32_t (&st)[Qua
                 wave_sync();
 #pragma unroll
                 for (int k = 0; k < PER; ++k) {
-                    if (!st[k] || rank[k] - r0 >= 64u) continue;
+                    if (!gen[k] || rank[k] - r0 >= 64u) continue;
                     const uint32_t res = stage[rank[k] - r0].x;
                     st[k] |= (res & 0xFu) << Q_UN;
                     sb[k] |= (res >> 4) << B_RT_SHIFT;

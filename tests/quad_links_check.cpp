@@ -4,7 +4,9 @@
 // following pixels with an equal left neighbour; its contacts are the
 // up-left / up / up-right equalities of those pixels; it links to the lowest
 // contact, else (pixel 0 continuing from the left quad) to the left pixel,
-// else to itself; other pixels get the last head at or before them.
+// else to itself; other pixels get the last head at or before them.  The
+// one-run form (quad_links_one_run) is checked the same way on every quad it
+// applies to (pixels 1..3 continue the run of pixel 0, pixel 0 a head).
 // Prints the number of mismatches; exit status 1 if any.
 #include <cstdio>
 
@@ -49,7 +51,7 @@ static void reference(uint32_t s, uint32_t p0, uint32_t T, uint32_t lab[4], uint
 
 int main() {
     const uint32_t T = 64, p0 = 5 * 64 + 8;
-    long cases = 0, bad = 0;
+    long cases = 0, one_run = 0, bad = 0;
     for (uint32_t v = 0; v < (1u << 20); ++v) {
         const uint32_t el = v & 0xFu, eu = (v >> 4) & 0xFu, eul = (v >> 8) & 0xFu, eur = (v >> 12) & 0xFu;
         const uint32_t nsg = (v >> 16) & 0xFu;
@@ -63,7 +65,16 @@ int main() {
         for (int i = 0; i < 4; ++i) ok = ok && a[i] == b[i];
         if (!ok && bad++ < 5)
             std::printf("mismatch el=%x eu=%x eul=%x eur=%x nsg=%x\n", el, eu, eul, eur, nsg);
+        if (q_one_run(s) && hd == 1u) {
+            uint32_t c[4], uc, rc;
+            quad_links_one_run(s, p0, T, c, uc, rc);
+            ++one_run;
+            bool ok1 = ua == uc && ra == rc;
+            for (int i = 0; i < 4; ++i) ok1 = ok1 && a[i] == c[i];
+            if (!ok1 && bad++ < 5)
+                std::printf("one-run mismatch el=%x eu=%x eul=%x eur=%x\n", el, eu, eul, eur);
+        }
     }
-    std::printf("%ld cases, %ld mismatches\n", cases, bad);
+    std::printf("%ld cases (%ld one-run), %ld mismatches\n", cases, one_run, bad);
     return bad ? 1 : 0;
 }

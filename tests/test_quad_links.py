@@ -20,7 +20,7 @@ def test_quad_links_exhaustive(tmp_path):
                    check=True, capture_output=True, text=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "1048576 cases, 0 mismatches" in r.stdout
+    assert "1048576 cases (65536 one-run), 0 mismatches" in r.stdout
 
 
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
