@@ -13,6 +13,7 @@
 // change, and only the tiles whose fragments did change are streamed again.
 #include "ss_device.hpp"
 #include "ss_internal.hpp"
+#include "ss_rowtile.hpp"
 #include "ss_tile.hpp"
 
 namespace ssx {
@@ -25,6 +26,16 @@ __global__ __launch_bounds__(NT) void k_tile_stage(Images ims, Ws ws) {
     if (gtile >= ws.ntiles) return;
     const ImgDev im = locate<MULTI>(ims, ws.tile_img, gtile);
     tile_stage<T, NT>(sm, im, gtile, ws);
+}
+
+// Row form (T = 64, 128 threads): one workgroup of two waves per tile.
+template <bool MULTI>
+__global__ __launch_bounds__(RT_THREADS) void k_row_tile(Images ims, Ws ws) {
+    __shared__ RowTileSmem sm;
+    const int64_t gtile = xcd_block(blockIdx.x, gridDim.x);
+    if (gtile >= ws.ntiles) return;
+    const ImgDev im = locate<MULTI>(ims, ws.tile_img, gtile);
+    row_tile_stage<MULTI>(sm, im, gtile, ws);
 }
 
 // Per-tile, per-merge-block and per-seam-unit image index of a batch: one
@@ -657,7 +668,10 @@ hipError_t launch_digest(const uint8_t* p, int64_t pitch, int w, int h, uint64_t
 // draining per-XCD queues needed 148 VGPRs per lane and ran slower).
 template <int T, int NT, bool MULTI>
 static void launch_tile_kernels(const Images& ims, const Ws& ws, hipStream_t s) {
-    k_tile_stage<T, NT, MULTI><<<(unsigned)ws.ntiles, NT, 0, s>>>(ims, ws);
+    if constexpr (T == RT && NT == RT_THREADS)
+        k_row_tile<MULTI><<<(unsigned)ws.ntiles, RT_THREADS, 0, s>>>(ims, ws);
+    else
+        k_tile_stage<T, NT, MULTI><<<(unsigned)ws.ntiles, NT, 0, s>>>(ims, ws);
 }
 
 template <int T, bool MULTI>

@@ -1,0 +1,587 @@
+// ss_rowtile.hpp -- row tile stage: the tile-local step of the pipeline for
+// T = 64 with 128 threads, formulated on 32-bit row masks.
+//
+// Mapping: one workgroup of two waves per 64 x 64 tile; wave h holds columns
+// 32h .. 32h + 31, lane y holds tile row y.  Bit b of a lane's masks is
+// column 32h + b of its row.
+//
+//   load      the tile plus its one-pixel halo into LDS (16-byte loads);
+//   classify  each lane reads its row and the rows above and below from LDS
+//             and computes, four pixels per dword, the greater / smaller
+//             evidence and its equal right / down / down-left / down-right
+//             neighbours; the up masks are the down masks of the lane above
+//             (DPP wave shift).  Seam bits: equal neighbours outside the tile
+//             and inside the image.
+//   runs      non-singleton pixels joined by equal right neighbours form
+//             runs; only a run's first pixel is a union-find node.  Unions
+//             (16-bit LDS parents, min-root CAS) follow row_edges: one per
+//             change of run above along a run, plus the three cross-half
+//             relations at columns 31 | 32 (ss_rows.hpp, checked on the host
+//             against a breadth-first labelling).
+//   evidence  a run's greater / smaller / seam bits reach its start by a
+//             backward fill along the run and are OR'ed into its root.
+//   output    seam roots are numbered (fragments); the masks are written
+//             (final for tile-contained zones, provisional for seam zones);
+//             if the tile holds an open fragment, its per-pixel record plane
+//             is built in LDS and streamed out; perimeter records and
+//             cross-seam equality bits as in the quad tile stage.
+//
+// The outputs (counts word, fragment evidence, perimeter records, cross-seam
+// bits, record plane, masks) are those of tile_stage in ss_tile.hpp, so every
+// later kernel is shared.
+#pragma once
+#include "ss_device.hpp"
+#include "ss_rows.hpp"
+#include "ss_tile.hpp"
+
+namespace ssx {
+
+constexpr int RT = 64;            // tile side
+constexpr int RT_THREADS = 128;   // two waves per tile
+constexpr int RT_P = 80;          // staged row pitch: tile column c of staged row r at r * RT_P + 16 + c
+constexpr int RT_ROWS = 66;       // staged rows: tile rows -1 .. 64
+
+// Label entries (16 bits): parent (bits 0-11) while the unions run; evidence
+// bits 12-14 OR'ed at roots; once numbered, a root holds RL_ROOT | evidence |
+// fragment id (RL_NOID if it is not a seam zone).
+constexpr uint32_t RL_IDX = 0x0FFFu;
+constexpr int RL_FSHIFT = 12;
+constexpr uint32_t RL_ROOT = 0x8000u;
+constexpr uint32_t RL_NOID = 0x0FFFu;
+
+struct RowTileSmem {
+    union {
+        alignas(16) uint8_t img[RT_ROWS * RT_P + 16];   // staged rows (+ the last row's right halo dword)
+        alignas(16) uint8_t plane[RT * RT];             // record plane, built after the staged rows are read
+    };
+    alignas(16) uint16_t lab[RT * RT];
+    uint32_t count, open, err, zflags;
+};
+
+__device__ __forceinline__ uint32_t rl_ld(const uint16_t* lab, uint32_t p) {
+    return __hip_atomic_load(lab + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void rl_st(uint16_t* lab, uint32_t p, uint32_t v) {
+    __hip_atomic_store(lab + p, (uint16_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t* rl_word(uint16_t* lab, uint32_t p) {
+    return reinterpret_cast<uint32_t*>(lab) + (p >> 1);
+}
+
+// Find with path halving (unions only: entries hold plain parents).
+__device__ __forceinline__ uint32_t rl_find_h(uint16_t* lab, uint32_t x) {
+    for (;;) {
+        const uint32_t p = rl_ld(lab, x);
+        const uint32_t gp = rl_ld(lab, p);
+        if (gp == p) return p;
+        rl_st(lab, x, gp);
+        x = gp;
+    }
+}
+
+// Min-root union.  The link is a CAS on the 32-bit word holding the larger
+// root's entry (the other half may change meanwhile: then the CAS is retried
+// as long as the entry is still a root); a root that got a parent meanwhile is
+// walked again from that parent, a strictly smaller index.
+__device__ __forceinline__ void rl_unite(uint16_t* lab, uint32_t a, uint32_t b) {
+    for (;;) {
+        a = rl_find_h(lab, a);
+        b = rl_find_h(lab, b);
+        if (a == b) return;
+        const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+        uint32_t* w = rl_word(lab, hi);
+        const int sh = (int)(hi & 1u) * 16;
+        uint32_t old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        for (;;) {
+            const uint32_t cur = (old >> sh) & 0xFFFFu;
+            if (cur != hi) break;
+            const uint32_t nw = (old & ~(0xFFFFu << sh)) | (lo << sh);
+            if (__hip_atomic_compare_exchange_strong(w, &old, nw, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP))
+                return;
+        }
+        a = (old >> sh) & RL_IDX;
+        b = lo;
+    }
+}
+
+// Read-only find (evidence bits masked).
+__device__ __forceinline__ uint32_t rl_find(const uint16_t* lab, uint32_t x) {
+    for (;;) {
+        const uint32_t p = rl_ld(lab, x) & RL_IDX;
+        if (p == x) return x;
+        x = p;
+    }
+}
+
+// Zone entry of a run start once the roots are numbered.
+__device__ __forceinline__ uint32_t rl_zone(const uint16_t* lab, uint32_t p) {
+    const uint32_t e = rl_ld(lab, p);
+    return (e & RL_ROOT) ? e : rl_ld(lab, e & RL_IDX);
+}
+
+// Lane y receives lane y - 1's value (lane 0: 0).
+__device__ __forceinline__ uint32_t from_lane_above(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, false);   // wave_shr:1
+}
+
+// Top bits of a 0x80-per-byte flag word -> 4-bit mask (bit k = byte k).
+__device__ __forceinline__ uint32_t nib80(uint32_t f) { return (f * 0x00204081u) >> 28; }
+// Bits 4d .. 4d + 3 of m -> 0x00 / 0xFF bytes.
+__device__ __forceinline__ uint32_t mask_bytes(uint32_t m, int d) {
+    return __builtin_amdgcn_perm(0u, 0u, 0x0C0C0C0Cu | spread4((m >> (4 * d)) & 0xFu));
+}
+__device__ __forceinline__ uint32_t eq80(uint32_t a, uint32_t b) { return ~ne_top(a, b) & 0x80808080u; }
+__device__ __forceinline__ uint32_t bit(uint32_t m, int b) { return (m >> b) & 1u; }
+
+// Staged rows: tile rows -1 .. th (clamped to the image, so an out-of-image
+// halo pixel replicates the nearest image pixel; see tile_load), columns
+// 0 .. tw - 1 and the halo columns -1 and tw.
+__device__ __forceinline__ void row_tile_load(RowTileSmem& sm, const ImgDev& im, int x0, int y0, int tw, int th) {
+    constexpr int NV = RT_ROWS * 4;                                  // 16-byte items
+    constexpr int ITEMS = (NV + RT_THREADS - 1) / RT_THREADS;
+    constexpr int HALO = (2 * RT_ROWS + RT_THREADS - 1) / RT_THREADS;
+    const int t = threadIdx.x;
+    if (x0 > 0 && y0 > 0 && x0 + RT < im.w && y0 + RT < im.h) {
+        const uint8_t* base = im.img + (int64_t)(y0 - 1) * im.pitch + x0;
+        uint4 v[ITEMS];
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int it = t + i * RT_THREADS;
+            v[i] = make_uint4(0, 0, 0, 0);
+            if (it < NV) __builtin_memcpy(&v[i], base + (int64_t)(it >> 2) * im.pitch + 16 * (it & 3), 16);
+        }
+        uint8_t hv[HALO];
+#pragma unroll
+        for (int i = 0; i < HALO; ++i) {
+            const int it = t + i * RT_THREADS;
+            hv[i] = 0;
+            if (it < 2 * RT_ROWS) hv[i] = base[(int64_t)(it >> 1) * im.pitch + ((it & 1) ? RT : -1)];
+        }
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int it = t + i * RT_THREADS;
+            if (it < NV) *reinterpret_cast<uint4*>(&sm.img[(it >> 2) * RT_P + 16 + 16 * (it & 3)]) = v[i];
+        }
+#pragma unroll
+        for (int i = 0; i < HALO; ++i) {
+            const int it = t + i * RT_THREADS;
+            if (it < 2 * RT_ROWS) sm.img[(it >> 1) * RT_P + 16 + ((it & 1) ? RT : -1)] = hv[i];
+        }
+        return;
+    }
+    auto row_of = [&](int r) -> const uint8_t* {   // staged row r = tile row r - 1
+        int gy = y0 + r - 1;
+        gy = gy < 0 ? 0 : (gy >= im.h ? im.h - 1 : gy);
+        return im.img + (int64_t)gy * im.pitch + x0;
+    };
+    const int nrows = th + 2;
+    uint4 v[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const int it = t + i * RT_THREADS;
+        const int r = it >> 2, c = 16 * (it & 3);
+        v[i] = make_uint4(0, 0, 0, 0);
+        if (r >= nrows || c >= tw) continue;
+        const uint8_t* src = row_of(r) + c;
+        if (c + 16 <= tw) {
+            __builtin_memcpy(&v[i], src, 16);
+        } else {
+            uint32_t q[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int b = 0; b < 16; ++b)
+                if (c + b < tw) q[b >> 2] |= (uint32_t)src[b] << (8 * (b & 3));
+            v[i] = make_uint4(q[0], q[1], q[2], q[3]);
+        }
+    }
+    uint8_t hv[HALO];
+#pragma unroll
+    for (int i = 0; i < HALO; ++i) {
+        const int it = t + i * RT_THREADS;
+        hv[i] = 0;
+        if ((it >> 1) >= nrows) continue;
+        int gx = x0 + ((it & 1) ? tw : -1);
+        gx = gx < 0 ? 0 : (gx >= im.w ? im.w - 1 : gx);
+        hv[i] = row_of(it >> 1)[gx - x0];
+    }
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const int it = t + i * RT_THREADS;
+        const int r = it >> 2, c = 16 * (it & 3);
+        if (r >= nrows || c >= tw) continue;
+        uint8_t* dst = &sm.img[r * RT_P + 16 + c];
+        if (c + 16 <= tw) {
+            *reinterpret_cast<uint4*>(dst) = v[i];
+        } else {
+            // bytes past tw stay untouched: column tw is the halo column
+            const uint32_t q[4] = {v[i].x, v[i].y, v[i].z, v[i].w};
+#pragma unroll
+            for (int b = 0; b < 16; ++b)
+                if (c + b < tw) dst[b] = (uint8_t)(q[b >> 2] >> (8 * (b & 3)));
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < HALO; ++i) {
+        const int it = t + i * RT_THREADS;
+        if ((it >> 1) >= nrows) continue;
+        sm.img[(it >> 1) * RT_P + 16 + ((it & 1) ? tw : -1)] = hv[i];
+    }
+}
+
+// A lane's view of one staged row: w[0] columns 32h - 4 .. 32h - 1, w[1..8]
+// columns 32h .. 32h + 31, w[9] columns 32h + 32 .. 32h + 35.
+struct HalfRow {
+    uint32_t w[10];
+};
+__device__ __forceinline__ HalfRow read_half_row(const RowTileSmem& sm, int r, int h) {
+    const uint8_t* p = sm.img + (r + 1) * RT_P + 16 + 32 * h;
+    HalfRow o;
+    o.w[0] = *reinterpret_cast<const uint32_t*>(p - 4);
+    const uint4 q0 = *reinterpret_cast<const uint4*>(p), q1 = *reinterpret_cast<const uint4*>(p + 16);
+    o.w[1] = q0.x; o.w[2] = q0.y; o.w[3] = q0.z; o.w[4] = q0.w;
+    o.w[5] = q1.x; o.w[6] = q1.y; o.w[7] = q1.z; o.w[8] = q1.w;
+    o.w[9] = *reinterpret_cast<const uint32_t*>(p + 32);
+    return o;
+}
+
+template <bool MULTI>
+__device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im, int64_t gtile, const Ws& ws) {
+    constexpr int S = 4 * RT - 4;
+    const int y = threadIdx.x & 63, h = threadIdx.x >> 6;
+    const uint32_t lt = (uint32_t)(gtile - im.tile_begin);
+    const int tx = (int)(lt % (uint32_t)im.tiles_x), ty = (int)(lt / (uint32_t)im.tiles_x);
+    const int x0 = tx * RT, y0 = ty * RT;
+    const int tw = min(RT, im.w - x0), th = min(RT, im.h - y0);
+    const bool hasL = x0 > 0, hasR = x0 + tw < im.w, hasU = y0 > 0, hasD = y0 + th < im.h;
+    if (threadIdx.x == 0) {
+        sm.count = 0;
+        sm.open = 0;
+        sm.err = 0;
+        sm.zflags = 0;
+    }
+    row_tile_load(sm, im, x0, y0, tw, th);
+    __syncthreads();
+
+    // ---- classification ---------------------------------------------------
+    const int nb = min(max(tw - 32 * h, 0), 32);   // valid columns of this half
+    const uint32_t VM = (y < th) ? (nb >= 32 ? 0xFFFFFFFFu : ((1u << nb) - 1u)) : 0u;
+    const HalfRow a = read_half_row(sm, y - 1, h), b = read_half_row(sm, y, h), c = read_half_row(sm, y + 1, h);
+    const uint32_t v4 = sm.img[RT_P + 16] * 0x01010101u;   // pixel (0, 0)
+    uint32_t G = 0, Sm = 0, ER = 0, ED = 0, EDL = 0, EDR = 0;
+    bool all_eq = true;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+        const RowMinMax ma = row_minmax(a.w[d], a.w[d + 1], a.w[d + 2]);
+        const RowMinMax mb = row_minmax(b.w[d], b.w[d + 1], b.w[d + 2]);
+        const RowMinMax mc = row_minmax(c.w[d], c.w[d + 1], c.w[d + 2]);
+        const uint32_t ge = pk_subsat(pk_max(pk_max(ma.maxe, mb.maxe), mc.maxe), mb.mid_e);
+        const uint32_t go = pk_subsat(pk_max(pk_max(ma.maxo, mb.maxo), mc.maxo), mb.mid_o);
+        const uint32_t se = pk_subsat(mb.mid_e, pk_min(pk_min(ma.mine, mb.mine), mc.mine));
+        const uint32_t so = pk_subsat(mb.mid_o, pk_min(pk_min(ma.mino, mb.mino), mc.mino));
+        const uint32_t g80 = (nz_top16(ge) >> 8) | nz_top16(go);
+        const uint32_t s80 = (nz_top16(se) >> 8) | nz_top16(so);
+        const uint32_t V = b.w[d + 1];
+        const uint32_t eR = eq80(V, __builtin_amdgcn_alignbyte(b.w[d + 2], V, 1));
+        const uint32_t eD = eq80(V, c.w[d + 1]);
+        const uint32_t eDL = eq80(V, __builtin_amdgcn_alignbyte(c.w[d + 1], c.w[d], 3));
+        const uint32_t eDR = eq80(V, __builtin_amdgcn_alignbyte(c.w[d + 2], c.w[d + 1], 1));
+        G |= nib80(g80) << (4 * d);
+        Sm |= nib80(s80) << (4 * d);
+        ER |= nib80(eR) << (4 * d);
+        ED |= nib80(eD) << (4 * d);
+        EDL |= nib80(eDL) << (4 * d);
+        EDR |= nib80(eDR) << (4 * d);
+        all_eq = all_eq && (ne_top(V, v4) & bytes4((VM >> (4 * d)) & 0xFu) & 0x80808080u) == 0;
+    }
+    G &= VM;
+    Sm &= VM;
+    // links inside the tile and the wave's half
+    const bool dv = y + 1 < th;
+    const uint32_t H = ER & VM & (VM >> 1);
+    const uint32_t D = dv ? ED & VM : 0u;
+    const uint32_t DL = dv ? EDL & VM & (VM << 1) : 0u;
+    const uint32_t DR = dv ? EDR & VM & (VM >> 1) : 0u;
+    const uint32_t U = from_lane_above(D), UL = from_lane_above(DR) << 1, UR = from_lane_above(DL) >> 1;
+    // relations across columns 31 | 32 (both pixels inside the tile)
+    const bool v31 = (VM >> 31) & 1u && tw > 32;   // wave 0: (y, 31) and (y, 32) valid
+    const bool v32 = h == 1 && (VM & 1u);          // wave 1: (y, 32) valid
+    const uint32_t XH = (h == 0 && v31) ? bit(ER, 31) : 0u;
+    const uint32_t XDR = (h == 0 && v31 && dv) ? bit(EDR, 31) : 0u;
+    const uint32_t XUR = (h == 0 && v31 && y > 0) ? (uint32_t)((b.w[8] >> 24) == (a.w[9] & 0xFFu)) : 0u;
+    uint32_t cross = (XH | XDR | XUR) << 31;
+    if (v32) {
+        const uint32_t p32 = b.w[1] & 0xFFu;
+        if (p32 == (b.w[0] >> 24) || (dv && bit(EDL, 0)) || (y > 0 && p32 == (a.w[0] >> 24))) cross |= 1u;
+    }
+    // seam bits: an equal 8-neighbour outside the tile and inside the image
+    uint32_t SEAM = 0;
+    {
+        // top row against the halo row: lane b of each wave takes column 32h + b
+        const int bl = y & 31, x = 32 * h + bl;
+        const uint8_t* r0 = sm.img + RT_P + 16;   // tile row 0
+        const uint8_t* rm = sm.img + 16;          // tile row -1
+        const uint32_t v = r0[x];
+        const bool in = y < 32 && x < tw && hasU;
+        const bool tu = rm[x] == v, tul = rm[x - 1] == v && (x > 0 || hasL), tur = rm[x + 1] == v && (x + 1 < tw || hasR);
+        const uint32_t top = (uint32_t)__ballot(in && (tu || tul || tur));
+        if (y == 0) SEAM |= top;
+    }
+    if (y == th - 1 && hasD) {
+        // bottom row against the halo row (the row below): the raw down masks
+        // corners: column -1 needs a tile on the left, column tw one on the right
+        const int br = tw - 1 - 32 * h;
+        const uint32_t rcorner = (!hasR && br >= 0 && br < 32) ? 1u << br : 0u;
+        SEAM |= (ED | (EDL & ~(h == 0 && !hasL ? 1u : 0u)) | (EDR & ~rcorner)) & VM;
+    }
+    if (h == 0 && hasL && (VM & 1u)) {
+        const uint32_t p0 = b.w[1] & 0xFFu;
+        if (p0 == (b.w[0] >> 24) || (p0 == (a.w[0] >> 24) && (y > 0 || hasU)) ||
+            (p0 == (c.w[0] >> 24) && (dv || hasD)))
+            SEAM |= 1u;
+    }
+    if (h == 1 && tw == 64 && hasR && (VM >> 31)) {
+        const uint32_t p63 = b.w[8] >> 24;
+        if (p63 == (b.w[9] & 0xFFu) || (p63 == (a.w[9] & 0xFFu) && (y > 0 || hasU)) ||
+            (p63 == (c.w[9] & 0xFFu) && (dv || hasD)))
+            SEAM |= 0x80000000u;
+    }
+    const uint32_t NS = VM & (H | (H << 1) | U | UL | UR | D | DL | DR | cross | SEAM);
+    const uint32_t RS = NS & ~(H << 1);
+    const uint32_t SGL = VM & ~NS;
+    const uint32_t pbase = (uint32_t)(64 * y + 32 * h);
+    // cross-seam equality bits (right column / bottom row)
+    uint32_t eqr = 0;
+    if (h == 1 && tw == 64 && hasR && y < th) {
+        const uint32_t p63 = b.w[8] >> 24;
+        eqr = (uint32_t)(p63 == (a.w[9] & 0xFFu) && (y > 0 || hasU)) | (uint32_t)(p63 == (b.w[9] & 0xFFu)) << 1 |
+              (uint32_t)(p63 == (c.w[9] & 0xFFu) && (dv || hasD)) << 2;
+    }
+    uint32_t eqb = 0;
+    if (th == 64 && hasD) {
+        const int bl = y & 31, x = 32 * h + bl;
+        const uint32_t ed = __builtin_amdgcn_readlane(ED, 63), edl = __builtin_amdgcn_readlane(EDL, 63),
+                       edr = __builtin_amdgcn_readlane(EDR, 63);
+        if (x < tw)
+            eqb = (bit(edl, bl) & (uint32_t)(x > 0 || hasL)) | bit(ed, bl) << 1 |
+                  (bit(edr, bl) & (uint32_t)(x + 1 < tw || hasR)) << 2;
+    }
+
+    const bool uniform = __syncthreads_and(all_eq) != 0;
+    uint8_t* rec = ws.rec + gtile * 4 * RT;
+    if (uniform) {
+        // one zone: OR the evidence, then everything is known
+        const uint32_t f = (G ? F_G : 0u) | (Sm ? F_S : 0u) | (SEAM ? F_SEAM : 0u);
+        const uint32_t wf = (__ballot(f & F_G) ? F_G : 0u) | (__ballot(f & F_S) ? F_S : 0u) |
+                            (__ballot(f & F_SEAM) ? F_SEAM : 0u);
+        if (y == 0 && wf) atomicOr(&sm.zflags, wf);
+        __syncthreads();
+        const uint32_t zf = sm.zflags;
+        if (threadIdx.x == 0 && (zf & F_SEAM)) {
+            sm.count = 1;
+            ws.sbits[gtile * S] = (uint8_t)(zf & 3u);
+        }
+        const uint32_t mxb = (zf & F_G) ? 0u : 0xFFFFFFFFu, mnb = (zf & F_S) ? 0u : 0xFFFFFFFFu;
+        if (y < th && nb > 0) {
+            const int64_t o = (int64_t)(y0 + y) * im.opitch + x0 + 32 * h;
+            if (nb == 32) {
+                store_u128(im.mx + o, make_uint4(mxb, mxb, mxb, mxb));
+                store_u128(im.mx + o + 16, make_uint4(mxb, mxb, mxb, mxb));
+                store_u128(im.mn + o, make_uint4(mnb, mnb, mnb, mnb));
+                store_u128(im.mn + o + 16, make_uint4(mnb, mnb, mnb, mnb));
+            } else {
+                for (int k = 0; k < nb; ++k) {
+                    im.mx[o + k] = (uint8_t)mxb;
+                    im.mn[o + k] = (uint8_t)mnb;
+                }
+            }
+        }
+        // records: fragment 0 everywhere on the perimeter
+        if (y < 32) {
+            rec[32 * h + y] = 0;
+            rec[RT + 32 * h + y] = 0;
+        }
+        if (h == 0) rec[2 * RT + y] = 0;
+        if (h == 1) rec[3 * RT + y] = 0;
+    } else {
+        // ---- runs: union-find over run starts ------------------------------
+        {
+            const uint32_t p0 = pbase * 0x00010001u + 0x00010000u;
+            uint4* dst = reinterpret_cast<uint4*>(sm.lab + pbase);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t k = p0 + (uint32_t)(8 * q) * 0x00010001u;
+                dst[q] = make_uint4(k, k + 0x00020002u, k + 0x00040004u, k + 0x00060006u);
+            }
+        }
+        __syncthreads();
+        const uint32_t RSup = from_lane_above(RS);
+        {
+            const RowEdges e = row_edges(U, UL, UR, H);
+            const uint32_t pup = pbase - 64u;
+            uint32_t todo = e.first;
+            while (todo) {
+                const int bb = __builtin_ctz(todo);
+                todo &= todo - 1u;
+                const int k = bb - (int)bit(e.kl, bb) + (int)bit(e.kr, bb);
+                rl_unite(sm.lab, pbase + high_at_or_below(RS, bb), pup + high_at_or_below(RSup, k));
+            }
+            todo = e.second;
+            while (todo) {
+                const int bb = __builtin_ctz(todo);
+                todo &= todo - 1u;
+                rl_unite(sm.lab, pbase + high_at_or_below(RS, bb), pup + high_at_or_below(RSup, bb + 1));
+            }
+            if (XH | XDR | XUR) {
+                const uint32_t a31 = pbase + high_at_or_below(RS, 31);
+                if (XH) rl_unite(sm.lab, a31, pbase + 32u);
+                if (XDR) rl_unite(sm.lab, a31, pbase + 64u + 32u);
+                if (XUR) rl_unite(sm.lab, a31, pbase - 64u + 32u);
+            }
+        }
+        __syncthreads();
+        // ---- evidence of each run at its root; compression -----------------
+        uint32_t roots = 0;
+        const uint32_t rfx = fill_bwd(SEAM, H) & RS;   // runs holding a seam pixel
+        {
+            const uint32_t rfg = fill_bwd(G & NS, H) & RS, rfs = fill_bwd(Sm & NS, H) & RS;
+            uint32_t m = RS;
+            while (m) {
+                const int bb = __builtin_ctz(m);
+                m &= m - 1u;
+                const uint32_t p = pbase + bb;
+                const uint32_t r = rl_find(sm.lab, p);
+                if (r == p) roots |= 1u << bb;
+                else rl_st(sm.lab, p, r);
+                const uint32_t f = bit(rfg, bb) | bit(rfs, bb) << 1 | bit(rfx, bb) << 2;
+                if (f && (f & ~(rl_ld(sm.lab, r) >> RL_FSHIFT))) {
+                    __hip_atomic_fetch_or(rl_word(sm.lab, r), f << (RL_FSHIFT + 16 * (r & 1u)), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+        }
+        __syncthreads();
+        // ---- number the seam zones (fragments) -----------------------------
+        {
+            uint32_t m = roots;
+            while (m) {
+                const int bb = __builtin_ctz(m);
+                m &= m - 1u;
+                const uint32_t p = pbase + bb;
+                const uint32_t f = (rl_ld(sm.lab, p) >> RL_FSHIFT) & 7u;
+                uint32_t id = RL_NOID;
+                if (f & F_SEAM) {
+                    const uint32_t j = atomicAdd(&sm.count, 1u);
+                    if (j >= (uint32_t)S) {
+                        sm.err = 1u;
+                    } else {
+                        id = j;
+                        ws.sbits[gtile * S + j] = (uint8_t)(f & 3u);
+                        // a fragment that already has both kinds of evidence is final
+                        if ((f & 3u) != 3u) sm.open = 1u;
+                    }
+                }
+                rl_st(sm.lab, p, RL_ROOT | f << RL_FSHIFT | id);
+            }
+        }
+        __syncthreads();
+        // ---- masks -----------------------------------------------------------
+        uint32_t mxs = 0, mns = 0, opens = 0;
+        {
+            uint32_t m = RS;
+            while (m) {
+                const int bb = __builtin_ctz(m);
+                m &= m - 1u;
+                const uint32_t p = pbase + bb;
+                const uint32_t e = rl_zone(sm.lab, p);
+                if (!((e >> RL_FSHIFT) & F_G)) mxs |= 1u << bb;
+                if (!((e >> RL_FSHIFT) & F_S)) mns |= 1u << bb;
+                if ((e & RL_IDX) != RL_NOID && ((e >> RL_FSHIFT) & 3u) != 3u) opens |= 1u << bb;
+                // a run holding a seam pixel whose zone got no fragment id
+                if (bit(rfx, bb) && (e & RL_IDX) == RL_NOID) sm.err = 2u;
+            }
+        }
+        const uint32_t MX = ((SGL & ~G) | fill_fwd(mxs, H)) & VM;
+        const uint32_t MN = ((SGL & ~Sm) | fill_fwd(mns, H)) & VM;
+        if (y < th && nb > 0) {
+            const int64_t o = (int64_t)(y0 + y) * im.opitch + x0 + 32 * h;
+            if (nb == 32) {
+                store_u128(im.mx + o, make_uint4(mask_bytes(MX, 0), mask_bytes(MX, 1), mask_bytes(MX, 2), mask_bytes(MX, 3)));
+                store_u128(im.mx + o + 16, make_uint4(mask_bytes(MX, 4), mask_bytes(MX, 5), mask_bytes(MX, 6), mask_bytes(MX, 7)));
+                store_u128(im.mn + o, make_uint4(mask_bytes(MN, 0), mask_bytes(MN, 1), mask_bytes(MN, 2), mask_bytes(MN, 3)));
+                store_u128(im.mn + o + 16, make_uint4(mask_bytes(MN, 4), mask_bytes(MN, 5), mask_bytes(MN, 6), mask_bytes(MN, 7)));
+            } else {
+                for (int k = 0; k < nb; ++k) {
+                    im.mx[o + k] = (uint8_t)(bit(MX, k) * 0xFFu);
+                    im.mn[o + k] = (uint8_t)(bit(MN, k) * 0xFFu);
+                }
+            }
+        }
+        // ---- perimeter records -------------------------------------------------
+        {
+            // top row (lanes 0-31) and bottom row (lanes 32-63), column 32h + (y & 31)
+            const int bl = y & 31, x = 32 * h + bl;
+            const int ry = y < 32 ? 0 : th - 1;
+            const uint32_t rs_r = __builtin_amdgcn_readlane(RS, 0), rs_b = __builtin_amdgcn_readlane(RS, th - 1);
+            const uint32_t ns_r = __builtin_amdgcn_readlane(NS, 0), ns_b = __builtin_amdgcn_readlane(NS, th - 1);
+            const uint32_t rsm = y < 32 ? rs_r : rs_b, nsm = y < 32 ? ns_r : ns_b;
+            if (x < tw && bit(nsm, bl)) {
+                const uint32_t e = rl_zone(sm.lab, (uint32_t)(64 * ry + 32 * h + high_at_or_below(rsm, bl)));
+                rec[(y < 32 ? 0 : RT) + x] = (uint8_t)(e & 0xFFu);
+            }
+            // left column (wave 0) and right column (the wave holding column tw - 1)
+            if (h == 0 && (NS & 1u)) rec[2 * RT + y] = (uint8_t)(rl_zone(sm.lab, pbase) & 0xFFu);
+            const int br = tw - 1 - 32 * h;
+            if (br >= 0 && br < 32 && bit(NS, br))
+                rec[3 * RT + y] = (uint8_t)(rl_zone(sm.lab, pbase + high_at_or_below(RS, br)) & 0xFFu);
+        }
+        // ---- record plane (tiles with an open fragment) ----------------------
+        if (sm.open) {
+            uint8_t* row = sm.plane + 64 * y + 32 * h;
+            uint32_t cw[8];
+#pragma unroll
+            for (int d = 0; d < 8; ++d)
+                cw[d] = REC_CLOSED * 0x01010101u | spread4((MX >> (4 * d)) & 0xFu) |
+                        spread4((MN >> (4 * d)) & 0xFu) << 1;
+            *reinterpret_cast<uint4*>(row) = make_uint4(cw[0], cw[1], cw[2], cw[3]);
+            *reinterpret_cast<uint4*>(row + 16) = make_uint4(cw[4], cw[5], cw[6], cw[7]);
+            uint32_t m = opens;
+            while (m) {
+                const int bb = __builtin_ctz(m);
+                m &= m - 1u;
+                const uint32_t id = rl_zone(sm.lab, pbase + bb) & 0xFFu;
+                // the run: bb .. the first column without an equal right link
+                const int be = __builtin_ctz(~H & (0xFFFFFFFFu << bb));
+                int q = bb;
+                while (q <= be) {
+                    if ((q & 3) == 0 && q + 3 <= be) {
+                        *reinterpret_cast<uint32_t*>(row + q) = id * 0x01010101u;
+                        q += 4;
+                    } else {
+                        row[q] = (uint8_t)id;
+                        q += 1;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t count = sm.count;
+    const bool open = !uniform && sm.open != 0;
+    if (open) {
+        uint8_t* plane = ws.rec8 + gtile * (int64_t)(RT * RT);
+        const uint4* src = reinterpret_cast<const uint4*>(sm.plane);
+        uint4* dst = reinterpret_cast<uint4*>(plane);
+#pragma unroll
+        for (int i = 0; i < RT * RT / 16 / RT_THREADS; ++i) dst[threadIdx.x + i * RT_THREADS] = src[threadIdx.x + i * RT_THREADS];
+    }
+    if (count) {
+        uint8_t* eq = ws.eq + gtile * 2 * RT;
+        if (h == 1) eq[y] = (uint8_t)eqr;
+        if (y < 32) eq[RT + 32 * h + y] = (uint8_t)eqb;
+    }
+    if (threadIdx.x == 0)
+        ws.counts[gtile] = min(count, (uint32_t)S) | (uniform ? CNT_UNIFORM : 0u) | (open ? CNT_OPEN : 0u) |
+                           (sm.err << COUNT_ERR_SHIFT);
+}
+
+}  // namespace ssx

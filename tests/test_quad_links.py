@@ -49,3 +49,18 @@ def test_autotune_admission_rule(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failed" in r.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_row_edges_partition(tmp_path):
+    """Host check of the row tile stage's union edges (ss_rows.hpp): on random
+    64 x 64 tiles the run union-find equals a breadth-first labelling of the
+    8-connected zones; the run fills equal a per-bit walk."""
+    exe = tmp_path / "rowtile_check"
+    src = os.path.join(REPO, "tests", "rowtile_check.cpp")
+    inc = os.path.join(REPO, "sparse-seam_amd", "csrc")
+    subprocess.run([HIPCC, "-x", "hip", "--offload-host-only", "-O2", "-std=c++17", "-I", inc, "-o", str(exe), src],
+                   check=True, capture_output=True, text=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "fills: 0 mismatches; 3000 tiles, 0 mismatches" in r.stdout
