@@ -13,7 +13,7 @@
 namespace ssx {
 
 static std::mutex g_cfg_mu;
-static ss_config g_default_cfg = {64, 256};
+static ss_config g_default_cfg = {64, 128};
 
 ss_config resolve_config(const ss_config* cfg) {
     if (cfg) return *cfg;

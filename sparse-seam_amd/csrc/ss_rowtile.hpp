@@ -41,6 +41,11 @@ constexpr int RT_THREADS = 128;   // two waves per tile
 constexpr int RT_P = 80;          // staged row pitch: tile column c of staged row r at r * RT_P + 16 + c
 constexpr int RT_ROWS = 66;       // staged rows: tile rows -1 .. 64
 
+// Union-find nodes are indexed column-major, id = 64 x + y for pixel (y, x),
+// so the entries the 64 lanes (rows) touch at one column are consecutive
+// halfwords, not one per 128-byte row (which all fall in the same LDS bank).
+__device__ __forceinline__ uint32_t rnode(int y, int x) { return (uint32_t)((x << 6) | y); }
+
 // Label entries (16 bits): parent (bits 0-11) while the unions run; evidence
 // bits 12-14 OR'ed at roots; once numbered, a root holds RL_ROOT | evidence |
 // fragment id (RL_NOID if it is not a seam zone).
@@ -52,7 +57,7 @@ constexpr uint32_t RL_NOID = 0x0FFFu;
 struct RowTileSmem {
     union {
         alignas(16) uint8_t img[RT_ROWS * RT_P + 16];   // staged rows (+ the last row's right halo dword)
-        alignas(16) uint8_t plane[RT * RT];             // record plane, built after the staged rows are read
+        alignas(16) uint8_t plane[RT * RT_P];           // record plane (row pitch RT_P), built after the staged rows are read
     };
     alignas(16) uint16_t lab[RT * RT];
     uint32_t count, open, err, zflags;
@@ -70,19 +75,22 @@ __device__ __forceinline__ uint32_t* rl_word(uint16_t* lab, uint32_t p) {
 
 // Find with path halving (unions only: entries hold plain parents).
 __device__ __forceinline__ uint32_t rl_find_h(uint16_t* lab, uint32_t x) {
+    uint32_t p = rl_ld(lab, x);
+    if (p == x) return x;
     for (;;) {
-        const uint32_t p = rl_ld(lab, x);
         const uint32_t gp = rl_ld(lab, p);
         if (gp == p) return p;
         rl_st(lab, x, gp);
         x = gp;
+        p = rl_ld(lab, x);
     }
 }
 
-// Min-root union.  The link is a CAS on the 32-bit word holding the larger
-// root's entry (the other half may change meanwhile: then the CAS is retried
-// as long as the entry is still a root); a root that got a parent meanwhile is
-// walked again from that parent, a strictly smaller index.
+// Union of two trees: the root with the larger index is linked under the
+// other by a CAS on the 32-bit word holding its entry (the other half may
+// change meanwhile: then the CAS is retried as long as the entry is still a
+// root); a root that got a parent meanwhile is walked again from that parent.
+// Every link joins two trees, so the parents stay a forest.
 __device__ __forceinline__ void rl_unite(uint16_t* lab, uint32_t a, uint32_t b) {
     for (;;) {
         a = rl_find_h(lab, a);
@@ -105,12 +113,18 @@ __device__ __forceinline__ void rl_unite(uint16_t* lab, uint32_t a, uint32_t b) 
     }
 }
 
-// Read-only find (evidence bits masked).
-__device__ __forceinline__ uint32_t rl_find(const uint16_t* lab, uint32_t x) {
+// Find with path halving once the roots carry evidence bits.  The unions
+// leave tall zones as chains of runs (each row linked to the one above), and
+// all lanes halve their walks at once, so a chain's depth halves per step.
+__device__ __forceinline__ uint32_t rl_find_hm(uint16_t* lab, uint32_t x) {
+    uint32_t p = rl_ld(lab, x) & RL_IDX;
+    if (p == x) return x;
     for (;;) {
-        const uint32_t p = rl_ld(lab, x) & RL_IDX;
-        if (p == x) return x;
-        x = p;
+        const uint32_t gp = rl_ld(lab, p) & RL_IDX;
+        if (gp == p) return p;
+        rl_st(lab, x, gp);   // x is not a root: its entry has no evidence bits
+        x = gp;
+        p = rl_ld(lab, x) & RL_IDX;
     }
 }
 
@@ -348,7 +362,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
     const uint32_t NS = VM & (H | (H << 1) | U | UL | UR | D | DL | DR | cross | SEAM);
     const uint32_t RS = NS & ~(H << 1);
     const uint32_t SGL = VM & ~NS;
-    const uint32_t pbase = (uint32_t)(64 * y + 32 * h);
+    const uint32_t nrow = rnode(y, 32 * h);   // node of column 32h of this row; column 32h + b is nrow + 64 b
     // cross-seam equality bits (right column / bottom row)
     uint32_t eqr = 0;
     if (h == 1 && tw == 64 && hasR && y < th) {
@@ -405,53 +419,65 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
     } else {
         // ---- runs: union-find over run starts ------------------------------
         {
-            const uint32_t p0 = pbase * 0x00010001u + 0x00010000u;
-            uint4* dst = reinterpret_cast<uint4*>(sm.lab + pbase);
+            // every entry its own parent, 16 bytes (8 entries) per thread and
+            // store, consecutive threads on consecutive bytes
+            uint4* dst = reinterpret_cast<uint4*>(sm.lab);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t k = p0 + (uint32_t)(8 * q) * 0x00010001u;
-                dst[q] = make_uint4(k, k + 0x00020002u, k + 0x00040004u, k + 0x00060006u);
+            for (int q = 0; q < RT * RT / 8 / RT_THREADS; ++q) {
+                const uint32_t c = threadIdx.x + q * RT_THREADS;
+                const uint32_t k = (8 * c) * 0x00010001u + 0x00010000u;
+                dst[c] = make_uint4(k, k + 0x00020002u, k + 0x00040004u, k + 0x00060006u);
             }
         }
         __syncthreads();
         const uint32_t RSup = from_lane_above(RS);
         {
             const RowEdges e = row_edges(U, UL, UR, H);
-            const uint32_t pup = pbase - 64u;
-            uint32_t todo = e.first;
+            const uint32_t nup = nrow - 1u;   // the row above
+            // A run's first edge is its initial link (Komura): a plain store of
+            // the run above as its parent.  Links always point one row up, so
+            // they form a forest; the other edges are unions on it.
+            const uint32_t first1 = e.first & ~((fill_fwd(e.first, H) & H) << 1);
+            uint32_t todo = first1;
             while (todo) {
                 const int bb = __builtin_ctz(todo);
                 todo &= todo - 1u;
                 const int k = bb - (int)bit(e.kl, bb) + (int)bit(e.kr, bb);
-                rl_unite(sm.lab, pbase + high_at_or_below(RS, bb), pup + high_at_or_below(RSup, k));
+                rl_st(sm.lab, nrow + (high_at_or_below(RS, bb) << 6), nup + (high_at_or_below(RSup, k) << 6));
+            }
+            __syncthreads();
+            todo = e.first & ~first1;
+            while (todo) {
+                const int bb = __builtin_ctz(todo);
+                todo &= todo - 1u;
+                const int k = bb - (int)bit(e.kl, bb) + (int)bit(e.kr, bb);
+                rl_unite(sm.lab, nrow + (high_at_or_below(RS, bb) << 6), nup + (high_at_or_below(RSup, k) << 6));
             }
             todo = e.second;
             while (todo) {
                 const int bb = __builtin_ctz(todo);
                 todo &= todo - 1u;
-                rl_unite(sm.lab, pbase + high_at_or_below(RS, bb), pup + high_at_or_below(RSup, bb + 1));
+                rl_unite(sm.lab, nrow + (high_at_or_below(RS, bb) << 6), nup + (high_at_or_below(RSup, bb + 1) << 6));
             }
             if (XH | XDR | XUR) {
-                const uint32_t a31 = pbase + high_at_or_below(RS, 31);
-                if (XH) rl_unite(sm.lab, a31, pbase + 32u);
-                if (XDR) rl_unite(sm.lab, a31, pbase + 64u + 32u);
-                if (XUR) rl_unite(sm.lab, a31, pbase - 64u + 32u);
+                const uint32_t a31 = nrow + (high_at_or_below(RS, 31) << 6);
+                if (XH) rl_unite(sm.lab, a31, rnode(y, 32));
+                if (XDR) rl_unite(sm.lab, a31, rnode(y + 1, 32));
+                if (XUR) rl_unite(sm.lab, a31, rnode(y - 1, 32));
             }
         }
         __syncthreads();
         // ---- evidence of each run at its root; compression -----------------
-        uint32_t roots = 0;
-        const uint32_t rfx = fill_bwd(SEAM, H) & RS;   // runs holding a seam pixel
         {
-            const uint32_t rfg = fill_bwd(G & NS, H) & RS, rfs = fill_bwd(Sm & NS, H) & RS;
+            const uint32_t rfg = fill_bwd(G & NS, H) & RS, rfs = fill_bwd(Sm & NS, H) & RS,
+                           rfx = fill_bwd(SEAM, H) & RS;
             uint32_t m = RS;
             while (m) {
                 const int bb = __builtin_ctz(m);
                 m &= m - 1u;
-                const uint32_t p = pbase + bb;
-                const uint32_t r = rl_find(sm.lab, p);
-                if (r == p) roots |= 1u << bb;
-                else rl_st(sm.lab, p, r);
+                const uint32_t p = nrow + (bb << 6);
+                const uint32_t r = rl_find_hm(sm.lab, p);
+                if (r != p) rl_st(sm.lab, p, r);
                 const uint32_t f = bit(rfg, bb) | bit(rfs, bb) << 1 | bit(rfx, bb) << 2;
                 if (f && (f & ~(rl_ld(sm.lab, r) >> RL_FSHIFT))) {
                     __hip_atomic_fetch_or(rl_word(sm.lab, r), f << (RL_FSHIFT + 16 * (r & 1u)), __ATOMIC_RELAXED,
@@ -460,46 +486,43 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
             }
         }
         __syncthreads();
-        // ---- number the seam zones (fragments) -----------------------------
-        {
-            uint32_t m = roots;
-            while (m) {
-                const int bb = __builtin_ctz(m);
-                m &= m - 1u;
-                const uint32_t p = pbase + bb;
-                const uint32_t f = (rl_ld(sm.lab, p) >> RL_FSHIFT) & 7u;
-                uint32_t id = RL_NOID;
-                if (f & F_SEAM) {
-                    const uint32_t j = atomicAdd(&sm.count, 1u);
-                    if (j >= (uint32_t)S) {
-                        sm.err = 1u;
-                    } else {
-                        id = j;
-                        ws.sbits[gtile * S + j] = (uint8_t)(f & 3u);
-                        // a fragment that already has both kinds of evidence is final
-                        if ((f & 3u) != 3u) sm.open = 1u;
-                    }
-                }
-                rl_st(sm.lab, p, RL_ROOT | f << RL_FSHIFT | id);
-            }
-        }
-        __syncthreads();
-        // ---- masks -----------------------------------------------------------
+        // ---- masks; the seam zones (fragments) are numbered on the way -----
+        // Every root is a run start, numbered by the lane that owns it (no
+        // other lane writes its entry); the other runs read the evidence from
+        // their root's entry, final whether it is numbered yet or not.
         uint32_t mxs = 0, mns = 0, opens = 0;
         {
             uint32_t m = RS;
             while (m) {
                 const int bb = __builtin_ctz(m);
                 m &= m - 1u;
-                const uint32_t p = pbase + bb;
-                const uint32_t e = rl_zone(sm.lab, p);
-                if (!((e >> RL_FSHIFT) & F_G)) mxs |= 1u << bb;
-                if (!((e >> RL_FSHIFT) & F_S)) mns |= 1u << bb;
-                if ((e & RL_IDX) != RL_NOID && ((e >> RL_FSHIFT) & 3u) != 3u) opens |= 1u << bb;
-                // a run holding a seam pixel whose zone got no fragment id
-                if (bit(rfx, bb) && (e & RL_IDX) == RL_NOID) sm.err = 2u;
+                const uint32_t p = nrow + (bb << 6);
+                uint32_t e = rl_ld(sm.lab, p);
+                if ((e & RL_IDX) == p) {
+                    const uint32_t f = (e >> RL_FSHIFT) & 7u;
+                    uint32_t id = RL_NOID;
+                    if (f & F_SEAM) {
+                        const uint32_t j = atomicAdd(&sm.count, 1u);
+                        if (j >= (uint32_t)S) {
+                            sm.err = 1u;
+                        } else {
+                            id = j;
+                            ws.sbits[gtile * S + j] = (uint8_t)(f & 3u);
+                        }
+                    }
+                    e = RL_ROOT | f << RL_FSHIFT | id;
+                    rl_st(sm.lab, p, e);
+                } else {
+                    e = rl_ld(sm.lab, e & RL_IDX);
+                }
+                const uint32_t f = (e >> RL_FSHIFT) & 7u;
+                if (!(f & F_G)) mxs |= 1u << bb;
+                if (!(f & F_S)) mns |= 1u << bb;
+                // a fragment that already has both kinds of evidence is final
+                if ((f & F_SEAM) && (f & 3u) != 3u) opens |= 1u << bb;
             }
         }
+        if (opens) sm.open = 1u;
         const uint32_t MX = ((SGL & ~G) | fill_fwd(mxs, H)) & VM;
         const uint32_t MN = ((SGL & ~Sm) | fill_fwd(mns, H)) & VM;
         if (y < th && nb > 0) {
@@ -516,27 +539,38 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                 }
             }
         }
+        __syncthreads();   // every root numbered
         // ---- perimeter records -------------------------------------------------
+        // A seam pixel's zone must have a fragment id (invariant code 2).
         {
             // top row (lanes 0-31) and bottom row (lanes 32-63), column 32h + (y & 31)
             const int bl = y & 31, x = 32 * h + bl;
             const int ry = y < 32 ? 0 : th - 1;
             const uint32_t rs_r = __builtin_amdgcn_readlane(RS, 0), rs_b = __builtin_amdgcn_readlane(RS, th - 1);
             const uint32_t ns_r = __builtin_amdgcn_readlane(NS, 0), ns_b = __builtin_amdgcn_readlane(NS, th - 1);
-            const uint32_t rsm = y < 32 ? rs_r : rs_b, nsm = y < 32 ? ns_r : ns_b;
+            const uint32_t sx_r = __builtin_amdgcn_readlane(SEAM, 0), sx_b = __builtin_amdgcn_readlane(SEAM, th - 1);
+            const uint32_t rsm = y < 32 ? rs_r : rs_b, nsm = y < 32 ? ns_r : ns_b, sxm = y < 32 ? sx_r : sx_b;
             if (x < tw && bit(nsm, bl)) {
-                const uint32_t e = rl_zone(sm.lab, (uint32_t)(64 * ry + 32 * h + high_at_or_below(rsm, bl)));
+                const uint32_t e = rl_zone(sm.lab, rnode(ry, 32 * h + high_at_or_below(rsm, bl)));
                 rec[(y < 32 ? 0 : RT) + x] = (uint8_t)(e & 0xFFu);
+                if (bit(sxm, bl) && (e & RL_IDX) == RL_NOID) sm.err = 2u;
             }
             // left column (wave 0) and right column (the wave holding column tw - 1)
-            if (h == 0 && (NS & 1u)) rec[2 * RT + y] = (uint8_t)(rl_zone(sm.lab, pbase) & 0xFFu);
+            if (h == 0 && (NS & 1u)) {
+                const uint32_t e = rl_zone(sm.lab, nrow);
+                rec[2 * RT + y] = (uint8_t)(e & 0xFFu);
+                if ((SEAM & 1u) && (e & RL_IDX) == RL_NOID) sm.err = 2u;
+            }
             const int br = tw - 1 - 32 * h;
-            if (br >= 0 && br < 32 && bit(NS, br))
-                rec[3 * RT + y] = (uint8_t)(rl_zone(sm.lab, pbase + high_at_or_below(RS, br)) & 0xFFu);
+            if (br >= 0 && br < 32 && bit(NS, br)) {
+                const uint32_t e = rl_zone(sm.lab, nrow + (high_at_or_below(RS, br) << 6));
+                rec[3 * RT + y] = (uint8_t)(e & 0xFFu);
+                if (bit(SEAM, br) && (e & RL_IDX) == RL_NOID) sm.err = 2u;
+            }
         }
         // ---- record plane (tiles with an open fragment) ----------------------
         if (sm.open) {
-            uint8_t* row = sm.plane + 64 * y + 32 * h;
+            uint8_t* row = sm.plane + RT_P * y + 32 * h;
             uint32_t cw[8];
 #pragma unroll
             for (int d = 0; d < 8; ++d)
@@ -548,7 +582,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
             while (m) {
                 const int bb = __builtin_ctz(m);
                 m &= m - 1u;
-                const uint32_t id = rl_zone(sm.lab, pbase + bb) & 0xFFu;
+                const uint32_t id = rl_zone(sm.lab, nrow + (bb << 6)) & 0xFFu;
                 // the run: bb .. the first column without an equal right link
                 const int be = __builtin_ctz(~H & (0xFFFFFFFFu << bb));
                 int q = bb;
@@ -569,10 +603,12 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
     const bool open = !uniform && sm.open != 0;
     if (open) {
         uint8_t* plane = ws.rec8 + gtile * (int64_t)(RT * RT);
-        const uint4* src = reinterpret_cast<const uint4*>(sm.plane);
         uint4* dst = reinterpret_cast<uint4*>(plane);
 #pragma unroll
-        for (int i = 0; i < RT * RT / 16 / RT_THREADS; ++i) dst[threadIdx.x + i * RT_THREADS] = src[threadIdx.x + i * RT_THREADS];
+        for (int i = 0; i < RT * RT / 16 / RT_THREADS; ++i) {
+            const int c = threadIdx.x + i * RT_THREADS;   // 16-byte chunk: row c / 4, part c % 4
+            dst[c] = *reinterpret_cast<const uint4*>(sm.plane + (c >> 2) * RT_P + 16 * (c & 3));
+        }
     }
     if (count) {
         uint8_t* eq = ws.eq + gtile * 2 * RT;

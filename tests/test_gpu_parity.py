@@ -112,11 +112,16 @@ def test_tile_counts_match_oracle(dev, tile):
         assert st["fragments"] == int(counts.sum())
 
 
+# Every tile size with its wide quad configuration, plus the row tile stage
+# (T = 64 with 128 threads, the default configuration).
+SCALE_CONFIGS = ((16, 256), (32, 256), (64, 256), (64, 128))
+
+
 def test_config1_image(dev):
     img = ssgen.config1()
     say("config1 512x512 sigma2 q32 seed0")
     emax, emin = oracle_global_uf(img.numpy())
-    for tile, threads in ((16, 256), (32, 256), (64, 256)):
+    for tile, threads in SCALE_CONFIGS:
         mx, mn, _, _ = run(img.numpy(), tile, threads)
         assert np.array_equal(mx, emax) and np.array_equal(mn, emin)
 
@@ -138,37 +143,38 @@ def test_sweep_small_sides(dev, side, seed):
     for name, img in ssgen.sweep_images(side, seed, device="cuda"):
         say("sweep side=%d seed=%d %s" % (side, seed, name))
         emax, emin = oracle_global_uf(img.cpu().numpy())
-        for tile in ss.TILES:
-            mx, mn, ws[tile] = run_dev(img, tile, 256, ws.get(tile))
-            assert np.array_equal(mx, emax), (name, tile)
-            assert np.array_equal(mn, emin), (name, tile)
+        for cfg in SCALE_CONFIGS:
+            mx, mn, ws[cfg] = run_dev(img, *cfg, ws.get(cfg))
+            assert np.array_equal(mx, emax), (name, cfg)
+            assert np.array_equal(mn, emin), (name, cfg)
 
 
 def test_adversarial_4096(dev):
     for tile in ss.TILES:
         for name, img in ssgen.adversarial_images(4096, tile, 0, device="cuda"):
-            say("adversarial 4096 tile=%d %s" % (tile, name))
             h = img.cpu().numpy()
             emax, emin = oracle_global_uf(h)
-            mx, mn, ws, st = run(h, tile, 256)
-            assert np.array_equal(mx, emax), (name, tile)
-            assert np.array_equal(mn, emin), (name, tile)
-            if name == "constant":
-                assert (mx == 255).all() and (mn == 255).all()
-            if name.startswith("serpentine"):
-                # the whole path is one zone with a greater neighbour at its start
-                assert not (mx[h == 254] == 255).any()
+            for threads in [n for t, n in SCALE_CONFIGS if t == tile]:
+                say("adversarial 4096 tile=%d threads=%d %s" % (tile, threads, name))
+                mx, mn, ws, st = run(h, tile, threads)
+                assert np.array_equal(mx, emax), (name, tile, threads)
+                assert np.array_equal(mn, emin), (name, tile, threads)
+                if name == "constant":
+                    assert (mx == 255).all() and (mn == 255).all()
+                if name.startswith("serpentine"):
+                    # the whole path is one zone with a greater neighbour at its start
+                    assert not (mx[h == 254] == 255).any()
 
 
 def test_seam_comb_fragment_capacity(dev):
     """Every perimeter pixel a fragment: tiles near 4T-4 fragments, and at
     T = 64 merge blocks with more fragments than the block merge keeps in LDS
     (its dense-array path)."""
-    for tile in ss.TILES:
+    for tile, threads in SCALE_CONFIGS:
         img = ssgen.seam_comb(1024, 1024, tile).numpy()
-        say("seam_comb 1024 tile=%d" % tile)
+        say("seam_comb 1024 tile=%d threads=%d" % (tile, threads))
         emax, emin, ecounts = oracle_sparse_seam(img, tile)
-        mx, mn, ws, st = run(img, tile, 256)
+        mx, mn, ws, st = run(img, tile, threads)
         assert np.array_equal(mx, emax) and np.array_equal(mn, emin), tile
         counts = ss.tile_counts(ws, *ecounts.shape)
         assert np.array_equal(counts, ecounts), tile
@@ -182,25 +188,26 @@ def test_large_sweep_vs_global_uf(dev, side):
     for name, img in ssgen.sweep_images(side, 0, device="cuda"):
         say("large sweep side=%d %s" % (side, name))
         emax, emin = oracle_global_uf(img.cpu().numpy())
-        for tile in ss.TILES:
-            mx, mn, ws[tile] = run_dev(img, tile, 256, ws.get(tile))
-            assert np.array_equal(mx, emax), (name, tile)
-            assert np.array_equal(mn, emin), (name, tile)
+        for cfg in SCALE_CONFIGS:
+            mx, mn, ws[cfg] = run_dev(img, *cfg, ws.get(cfg))
+            assert np.array_equal(mx, emax), (name, cfg)
+            assert np.array_equal(mn, emin), (name, cfg)
 
 
-@pytest.mark.parametrize("tile", [64, 32, 16])
-def test_full_size_sweep_vs_global_uf(dev, tile):
+def test_full_size_sweep_vs_global_uf(dev):
     """The benchmark's workload at full size: every image of the
-    16384x16384 sweep at each tile size (T = 64 is the benchmark's default
-    configuration), bit-exact against the CPU Global-UF route (one image is a
-    second or two of CPU work on the host's cores)."""
-    ws = None
+    16384x16384 sweep in every configuration above (T = 64 with 128 threads is
+    the benchmark's default), bit-exact against the CPU Global-UF route (one
+    image is a second or two of CPU work on the host's cores, done once per
+    image)."""
+    ws = {}
     for name, img in ssgen.sweep_images(16384, 0, device="cuda"):
-        say("full-size sweep 16384 tile=%d %s" % (tile, name))
-        mx, mn, ws = run_dev(img, tile, 256, ws)
         emax, emin = oracle_global_uf(img.cpu().numpy())
-        assert np.array_equal(mx, emax), name
-        assert np.array_equal(mn, emin), name
+        for cfg in SCALE_CONFIGS:
+            say("full-size sweep 16384 tile=%d threads=%d %s" % (cfg[0], cfg[1], name))
+            mx, mn, ws[cfg] = run_dev(img, *cfg, ws.get(cfg))
+            assert np.array_equal(mx, emax), (name, cfg)
+            assert np.array_equal(mn, emin), (name, cfg)
 
 
 def test_many_tiles_per_writeback_wave(dev):
@@ -226,19 +233,23 @@ def test_strided_rows_and_outputs(dev):
     view = base[:, 17:17 + 211]            # pitch 300, width 211
     omx = torch.full((130, 256), 7, dtype=torch.uint8, device="cuda")
     omn = torch.full((130, 256), 7, dtype=torch.uint8, device="cuda")
-    mx, mn, ws = ss.extrema(view, 32, 256, out=(omx[:, :211], omn[:, :211]))
-    torch.cuda.synchronize()
     emax, emin = oracle_global_uf(view.cpu().numpy())
-    assert np.array_equal(mx.cpu().numpy(), emax) and np.array_equal(mn.cpu().numpy(), emin)
-    assert (omx[:, 211:] == 7).all() and (omn[:, 211:] == 7).all()
+    for tile, threads in ((32, 256), (64, 128)):
+        omx.fill_(7)
+        omn.fill_(7)
+        mx, mn, ws = ss.extrema(view, tile, threads, out=(omx[:, :211], omn[:, :211]))
+        torch.cuda.synchronize()
+        assert np.array_equal(mx.cpu().numpy(), emax) and np.array_equal(mn.cpu().numpy(), emin)
+        assert (omx[:, 211:] == 7).all() and (omn[:, 211:] == 7).all()
 
 
 def test_repeat_runs_are_identical(dev):
     img = ssgen.smooth(1000, 1500, 4.0, 4, 2, device="cuda")
     ref = None
-    ws = None
-    for _ in range(5):
-        mx, mn, ws = ss.extrema(img, 32, 256, ws)
+    ws = {}
+    for k in range(6):
+        cfg = (32, 256) if k % 2 else (64, 128)
+        mx, mn, ws[cfg] = ss.extrema(img, *cfg, ws.get(cfg))
         cur = (mx.cpu().numpy(), mn.cpu().numpy())
         if ref is None:
             ref = cur
