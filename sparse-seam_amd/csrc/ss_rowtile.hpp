@@ -280,32 +280,28 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
     const int nb = min(max(tw - 32 * h, 0), 32);   // valid columns of this half
     const uint32_t VM = (y < th) ? (nb >= 32 ? 0xFFFFFFFFu : ((1u << nb) - 1u)) : 0u;
     const HalfRow a = read_half_row(sm, y - 1, h), b = read_half_row(sm, y, h), c = read_half_row(sm, y + 1, h);
-    const uint32_t v4 = sm.img[RT_P + 16] * 0x01010101u;   // pixel (0, 0)
     uint32_t G = 0, Sm = 0, ER = 0, ED = 0, EDL = 0, EDR = 0;
-    bool all_eq = true;
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
         const RowMinMax ma = row_minmax(a.w[d], a.w[d + 1], a.w[d + 2]);
         const RowMinMax mb = row_minmax(b.w[d], b.w[d + 1], b.w[d + 2]);
         const RowMinMax mc = row_minmax(c.w[d], c.w[d + 1], c.w[d + 2]);
-        const uint32_t ge = pk_subsat(pk_max(pk_max(ma.maxe, mb.maxe), mc.maxe), mb.mid_e);
-        const uint32_t go = pk_subsat(pk_max(pk_max(ma.maxo, mb.maxo), mc.maxo), mb.mid_o);
-        const uint32_t se = pk_subsat(mb.mid_e, pk_min(pk_min(ma.mine, mb.mine), mc.mine));
-        const uint32_t so = pk_subsat(mb.mid_o, pk_min(pk_min(ma.mino, mb.mino), mc.mino));
-        const uint32_t g80 = (nz_top16(ge) >> 8) | nz_top16(go);
-        const uint32_t s80 = (nz_top16(se) >> 8) | nz_top16(so);
+        // 3 x 3 maximum and minimum (the pixel itself included) back in byte
+        // form: a strictly greater neighbour exists iff the maximum differs
+        // from the pixel, a strictly smaller one iff the minimum does
+        const uint32_t mx3 = pk_max(pk_max(ma.maxe, mb.maxe), mc.maxe) | pk_max(pk_max(ma.maxo, mb.maxo), mc.maxo) << 8;
+        const uint32_t mn3 = pk_min(pk_min(ma.mine, mb.mine), mc.mine) | pk_min(pk_min(ma.mino, mb.mino), mc.mino) << 8;
         const uint32_t V = b.w[d + 1];
         const uint32_t eR = eq80(V, __builtin_amdgcn_alignbyte(b.w[d + 2], V, 1));
         const uint32_t eD = eq80(V, c.w[d + 1]);
         const uint32_t eDL = eq80(V, __builtin_amdgcn_alignbyte(c.w[d + 1], c.w[d], 3));
         const uint32_t eDR = eq80(V, __builtin_amdgcn_alignbyte(c.w[d + 2], c.w[d + 1], 1));
-        G |= nib80(g80) << (4 * d);
-        Sm |= nib80(s80) << (4 * d);
+        G |= nib80(ne_top(mx3, V)) << (4 * d);
+        Sm |= nib80(ne_top(mn3, V)) << (4 * d);
         ER |= nib80(eR) << (4 * d);
         ED |= nib80(eD) << (4 * d);
         EDL |= nib80(eDL) << (4 * d);
         EDR |= nib80(eDR) << (4 * d);
-        all_eq = all_eq && (ne_top(V, v4) & bytes4((VM >> (4 * d)) & 0xFu) & 0x80808080u) == 0;
     }
     G &= VM;
     Sm &= VM;
@@ -380,6 +376,9 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                   (bit(edr, bl) & (uint32_t)(x + 1 < tw || hasR)) << 2;
     }
 
+    // a uniform tile (one value): every valid pixel equals its right and lower
+    // neighbours inside the tile, and the halves meet equal
+    const bool all_eq = H == (VM & (VM >> 1)) && (!dv || D == VM) && (h == 1 || !v31 || XH);
     const bool uniform = __syncthreads_and(all_eq) != 0;
     uint8_t* rec = ws.rec + gtile * 4 * RT;
     if (uniform) {
@@ -468,6 +467,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         }
         __syncthreads();
         // ---- evidence of each run at its root; compression -----------------
+        uint32_t roots = 0;
         {
             const uint32_t rfg = fill_bwd(G & NS, H) & RS, rfs = fill_bwd(Sm & NS, H) & RS,
                            rfx = fill_bwd(SEAM, H) & RS;
@@ -476,8 +476,21 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                 const int bb = __builtin_ctz(m);
                 m &= m - 1u;
                 const uint32_t p = nrow + (bb << 6);
-                const uint32_t r = rl_find_hm(sm.lab, p);
-                if (r != p) rl_st(sm.lab, p, r);
+                // find with path halving, one exit: at a root the grandparent
+                // is the parent itself
+                uint32_t x = p, r;
+                for (;;) {
+                    const uint32_t px = rl_ld(sm.lab, x) & RL_IDX;
+                    const uint32_t gp = rl_ld(sm.lab, px) & RL_IDX;
+                    if (gp == px) {
+                        r = px;
+                        break;
+                    }
+                    rl_st(sm.lab, x, gp);   // x is not a root: its entry has no evidence bits
+                    x = gp;
+                }
+                if (r == p) roots |= 1u << bb;
+                else rl_st(sm.lab, p, r);
                 const uint32_t f = bit(rfg, bb) | bit(rfs, bb) << 1 | bit(rfx, bb) << 2;
                 if (f && (f & ~(rl_ld(sm.lab, r) >> RL_FSHIFT))) {
                     __hip_atomic_fetch_or(rl_word(sm.lab, r), f << (RL_FSHIFT + 16 * (r & 1u)), __ATOMIC_RELAXED,
@@ -486,42 +499,50 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
             }
         }
         __syncthreads();
-        // ---- masks; the seam zones (fragments) are numbered on the way -----
-        // Every root is a run start, numbered by the lane that owns it (no
-        // other lane writes its entry); the other runs read the evidence from
-        // their root's entry, final whether it is numbered yet or not.
-        uint32_t mxs = 0, mns = 0, opens = 0;
+        // ---- number the seam zones (fragments) -----------------------------
+        // A root's entry becomes RL_ROOT | evidence | id; only its own lane
+        // writes it, and the evidence bits stay, so the mask pass below reads
+        // the evidence of every zone whether its root is numbered yet or not.
+        {
+            uint32_t m = roots;
+            while (m) {
+                const int bb = __builtin_ctz(m);
+                m &= m - 1u;
+                const uint32_t p = nrow + (bb << 6);
+                const uint32_t f = (rl_ld(sm.lab, p) >> RL_FSHIFT) & 7u;
+                uint32_t id = RL_NOID;
+                if (f & F_SEAM) {
+                    const uint32_t j = atomicAdd(&sm.count, 1u);
+                    if (j >= (uint32_t)S) {
+                        sm.err = 1u;
+                    } else {
+                        id = j;
+                        ws.sbits[gtile * S + j] = (uint8_t)(f & 3u);
+                    }
+                }
+                rl_st(sm.lab, p, RL_ROOT | f << RL_FSHIFT | id);
+            }
+        }
+        // ---- masks: each run takes its zone's evidence ----------------------
+        uint32_t zg = 0, zs = 0, zx = 0;   // at run starts: the zone's G / S / seam bits
         {
             uint32_t m = RS;
             while (m) {
                 const int bb = __builtin_ctz(m);
                 m &= m - 1u;
                 const uint32_t p = nrow + (bb << 6);
-                uint32_t e = rl_ld(sm.lab, p);
-                if ((e & RL_IDX) == p) {
-                    const uint32_t f = (e >> RL_FSHIFT) & 7u;
-                    uint32_t id = RL_NOID;
-                    if (f & F_SEAM) {
-                        const uint32_t j = atomicAdd(&sm.count, 1u);
-                        if (j >= (uint32_t)S) {
-                            sm.err = 1u;
-                        } else {
-                            id = j;
-                            ws.sbits[gtile * S + j] = (uint8_t)(f & 3u);
-                        }
-                    }
-                    e = RL_ROOT | f << RL_FSHIFT | id;
-                    rl_st(sm.lab, p, e);
-                } else {
-                    e = rl_ld(sm.lab, e & RL_IDX);
-                }
-                const uint32_t f = (e >> RL_FSHIFT) & 7u;
-                if (!(f & F_G)) mxs |= 1u << bb;
-                if (!(f & F_S)) mns |= 1u << bb;
-                // a fragment that already has both kinds of evidence is final
-                if ((f & F_SEAM) && (f & 3u) != 3u) opens |= 1u << bb;
+                const uint32_t e = rl_ld(sm.lab, p);
+                // the root's entry (read unconditionally: an index below 4096 either way)
+                const uint32_t er = rl_ld(sm.lab, e & RL_IDX);
+                const uint32_t z = ((e & RL_ROOT) || (e & RL_IDX) == p) ? e : er;
+                zg |= ((z >> RL_FSHIFT) & 1u) << bb;
+                zs |= ((z >> (RL_FSHIFT + 1)) & 1u) << bb;
+                zx |= ((z >> (RL_FSHIFT + 2)) & 1u) << bb;
             }
         }
+        const uint32_t mxs = RS & ~zg, mns = RS & ~zs;
+        // a fragment that already has both kinds of evidence is final
+        const uint32_t opens = zx & ~(zg & zs);
         if (opens) sm.open = 1u;
         const uint32_t MX = ((SGL & ~G) | fill_fwd(mxs, H)) & VM;
         const uint32_t MN = ((SGL & ~Sm) | fill_fwd(mns, H)) & VM;
