@@ -471,31 +471,45 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         {
             const uint32_t rfg = fill_bwd(G & NS, H) & RS, rfs = fill_bwd(Sm & NS, H) & RS,
                            rfx = fill_bwd(SEAM, H) & RS;
+            // two runs per iteration (the second repeats the first when the row
+            // has none left), their two finds walked together: the dependent
+            // LDS loads of one overlap those of the other
             uint32_t m = RS;
             while (m) {
-                const int bb = __builtin_ctz(m);
+                const int b1 = __builtin_ctz(m);
                 m &= m - 1u;
-                const uint32_t p = nrow + (bb << 6);
+                const int b2 = m ? __builtin_ctz(m) : b1;
+                m &= m - 1u;
+                const uint32_t p1 = nrow + (b1 << 6), p2 = nrow + (b2 << 6);
                 // find with path halving, one exit: at a root the grandparent
                 // is the parent itself
-                uint32_t x = p, r;
+                uint32_t x1 = p1, x2 = p2;
                 for (;;) {
-                    const uint32_t px = rl_ld(sm.lab, x) & RL_IDX;
-                    const uint32_t gp = rl_ld(sm.lab, px) & RL_IDX;
-                    if (gp == px) {
-                        r = px;
+                    const uint32_t q1 = rl_ld(sm.lab, x1) & RL_IDX, q2 = rl_ld(sm.lab, x2) & RL_IDX;
+                    const uint32_t g1 = rl_ld(sm.lab, q1) & RL_IDX, g2 = rl_ld(sm.lab, q2) & RL_IDX;
+                    const bool d1 = g1 == q1, d2 = g2 == q2;
+                    if (d1 && d2) {
+                        x1 = q1;
+                        x2 = q2;
                         break;
                     }
-                    rl_st(sm.lab, x, gp);   // x is not a root: its entry has no evidence bits
-                    x = gp;
+                    // a node that is not a root has no evidence bits
+                    if (!d1) { rl_st(sm.lab, x1, g1); x1 = g1; }
+                    if (!d2) { rl_st(sm.lab, x2, g2); x2 = g2; }
                 }
-                if (r == p) roots |= 1u << bb;
-                else rl_st(sm.lab, p, r);
-                const uint32_t f = bit(rfg, bb) | bit(rfs, bb) << 1 | bit(rfx, bb) << 2;
-                if (f && (f & ~(rl_ld(sm.lab, r) >> RL_FSHIFT))) {
-                    __hip_atomic_fetch_or(rl_word(sm.lab, r), f << (RL_FSHIFT + 16 * (r & 1u)), __ATOMIC_RELAXED,
+                if (x1 == p1) roots |= 1u << b1;
+                else rl_st(sm.lab, p1, x1);
+                if (x2 == p2) roots |= 1u << b2;
+                else rl_st(sm.lab, p2, x2);
+                const uint32_t f1 = bit(rfg, b1) | bit(rfs, b1) << 1 | bit(rfx, b1) << 2;
+                const uint32_t f2 = bit(rfg, b2) | bit(rfs, b2) << 1 | bit(rfx, b2) << 2;
+                const uint32_t h1 = rl_ld(sm.lab, x1) >> RL_FSHIFT, h2 = rl_ld(sm.lab, x2) >> RL_FSHIFT;
+                if (f1 & ~h1)
+                    __hip_atomic_fetch_or(rl_word(sm.lab, x1), f1 << (RL_FSHIFT + 16 * (x1 & 1u)), __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                }
+                if (f2 & ~h2)
+                    __hip_atomic_fetch_or(rl_word(sm.lab, x2), f2 << (RL_FSHIFT + 16 * (x2 & 1u)), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
         __syncthreads();
@@ -528,16 +542,19 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         {
             uint32_t m = RS;
             while (m) {
-                const int bb = __builtin_ctz(m);
+                const int b1 = __builtin_ctz(m);
                 m &= m - 1u;
-                const uint32_t p = nrow + (bb << 6);
-                const uint32_t e = rl_ld(sm.lab, p);
-                // the root's entry (read unconditionally: an index below 4096 either way)
-                const uint32_t er = rl_ld(sm.lab, e & RL_IDX);
-                const uint32_t z = ((e & RL_ROOT) || (e & RL_IDX) == p) ? e : er;
-                zg |= ((z >> RL_FSHIFT) & 1u) << bb;
-                zs |= ((z >> (RL_FSHIFT + 1)) & 1u) << bb;
-                zx |= ((z >> (RL_FSHIFT + 2)) & 1u) << bb;
+                const int b2 = m ? __builtin_ctz(m) : b1;
+                m &= m - 1u;
+                const uint32_t p1 = nrow + (b1 << 6), p2 = nrow + (b2 << 6);
+                const uint32_t e1 = rl_ld(sm.lab, p1), e2 = rl_ld(sm.lab, p2);
+                // the roots' entries (read unconditionally: an index below 4096 either way)
+                const uint32_t r1 = rl_ld(sm.lab, e1 & RL_IDX), r2 = rl_ld(sm.lab, e2 & RL_IDX);
+                const uint32_t z1 = ((e1 & RL_ROOT) || (e1 & RL_IDX) == p1) ? e1 : r1;
+                const uint32_t z2 = ((e2 & RL_ROOT) || (e2 & RL_IDX) == p2) ? e2 : r2;
+                zg |= ((z1 >> RL_FSHIFT) & 1u) << b1 | ((z2 >> RL_FSHIFT) & 1u) << b2;
+                zs |= ((z1 >> (RL_FSHIFT + 1)) & 1u) << b1 | ((z2 >> (RL_FSHIFT + 1)) & 1u) << b2;
+                zx |= ((z1 >> (RL_FSHIFT + 2)) & 1u) << b1 | ((z2 >> (RL_FSHIFT + 2)) & 1u) << b2;
             }
         }
         const uint32_t mxs = RS & ~zg, mns = RS & ~zs;
