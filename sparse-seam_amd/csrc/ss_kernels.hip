@@ -28,14 +28,17 @@ __global__ __launch_bounds__(NT) void k_tile_stage(Images ims, Ws ws) {
     tile_stage<T, NT>(sm, im, gtile, ws);
 }
 
-// Row form (T = 64, 128 threads): one workgroup of two waves per tile.
+// Row form (T = 64, 128 threads per tile): a workgroup holds RT_NTW
+// consecutive tiles (horizontal neighbours within an image row), two waves
+// each, so that its loads and stores cover whole 128-byte lines.
 template <bool MULTI>
-__global__ __launch_bounds__(RT_THREADS) void k_row_tile(Images ims, Ws ws) {
-    __shared__ RowTileSmem sm;
-    const int64_t gtile = xcd_block(blockIdx.x, gridDim.x);
-    if (gtile >= ws.ntiles) return;
-    const ImgDev im = locate<MULTI>(ims, ws.tile_img, gtile);
-    row_tile_stage<MULTI>(sm, im, gtile, ws);
+__global__ __launch_bounds__(RT_THREADS * RT_NTW) void k_row_tile(Images ims, Ws ws) {
+    __shared__ RowTileSmem sm[RT_NTW];
+    const int sub = threadIdx.x / RT_THREADS;
+    const int64_t gtile = xcd_block(blockIdx.x, gridDim.x) * RT_NTW + sub;
+    const bool live = gtile < ws.ntiles;
+    const ImgDev im = live ? locate<MULTI>(ims, ws.tile_img, gtile) : ims.one;
+    row_tile_stage<MULTI>(sm[sub], im, gtile, ws, live, threadIdx.x % RT_THREADS);
 }
 
 // Per-tile, per-merge-block and per-seam-unit image index of a batch: one
@@ -669,7 +672,7 @@ hipError_t launch_digest(const uint8_t* p, int64_t pitch, int w, int h, uint64_t
 template <int T, int NT, bool MULTI>
 static void launch_tile_kernels(const Images& ims, const Ws& ws, hipStream_t s) {
     if constexpr (T == RT && NT == RT_THREADS)
-        k_row_tile<MULTI><<<(unsigned)ws.ntiles, RT_THREADS, 0, s>>>(ims, ws);
+        k_row_tile<MULTI><<<(unsigned)((ws.ntiles + RT_NTW - 1) / RT_NTW), RT_THREADS * RT_NTW, 0, s>>>(ims, ws);
     else
         k_tile_stage<T, NT, MULTI><<<(unsigned)ws.ntiles, NT, 0, s>>>(ims, ws);
 }

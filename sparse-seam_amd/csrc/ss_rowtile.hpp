@@ -38,6 +38,7 @@ namespace ssx {
 
 constexpr int RT = 64;            // tile side
 constexpr int RT_THREADS = 128;   // two waves per tile
+constexpr int RT_NTW = 1;         // tiles per workgroup
 constexpr int RT_P = 80;          // staged row pitch: tile column c of staged row r at r * RT_P + 16 + c
 constexpr int RT_ROWS = 66;       // staged rows: tile rows -1 .. 64
 
@@ -60,7 +61,7 @@ struct RowTileSmem {
         alignas(16) uint8_t plane[RT * RT_P];           // record plane (row pitch RT_P), built after the staged rows are read
     };
     alignas(16) uint16_t lab[RT * RT];
-    uint32_t count, open, err, zflags;
+    uint32_t count, open, err, zflags, nonuni;
 };
 
 __device__ __forceinline__ uint32_t rl_ld(const uint16_t* lab, uint32_t p) {
@@ -151,11 +152,11 @@ __device__ __forceinline__ uint32_t bit(uint32_t m, int b) { return (m >> b) & 1
 // Staged rows: tile rows -1 .. th (clamped to the image, so an out-of-image
 // halo pixel replicates the nearest image pixel; see tile_load), columns
 // 0 .. tw - 1 and the halo columns -1 and tw.
-__device__ __forceinline__ void row_tile_load(RowTileSmem& sm, const ImgDev& im, int x0, int y0, int tw, int th) {
+__device__ __forceinline__ void row_tile_load(RowTileSmem& sm, const ImgDev& im, int x0, int y0, int tw, int th,
+                                              int t) {
     constexpr int NV = RT_ROWS * 4;                                  // 16-byte items
     constexpr int ITEMS = (NV + RT_THREADS - 1) / RT_THREADS;
     constexpr int HALO = (2 * RT_ROWS + RT_THREADS - 1) / RT_THREADS;
-    const int t = threadIdx.x;
     if (x0 > 0 && y0 > 0 && x0 + RT < im.w && y0 + RT < im.h) {
         const uint8_t* base = im.img + (int64_t)(y0 - 1) * im.pitch + x0;
         uint4 v[ITEMS];
@@ -259,21 +260,27 @@ __device__ __forceinline__ HalfRow read_half_row(const RowTileSmem& sm, int r, i
 }
 
 template <bool MULTI>
-__device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im, int64_t gtile, const Ws& ws) {
+__device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im, int64_t gtile, const Ws& ws,
+                                               bool live, int tid) {
     constexpr int S = 4 * RT - 4;
-    const int y = threadIdx.x & 63, h = threadIdx.x >> 6;
-    const uint32_t lt = (uint32_t)(gtile - im.tile_begin);
-    const int tx = (int)(lt % (uint32_t)im.tiles_x), ty = (int)(lt / (uint32_t)im.tiles_x);
-    const int x0 = tx * RT, y0 = ty * RT;
-    const int tw = min(RT, im.w - x0), th = min(RT, im.h - y0);
+    const int y = tid & 63, h = tid >> 6;
+    int x0 = 0, y0 = 0, tw = 0, th = 0;   // a workgroup's missing last tile has no pixels
+    if (live) {
+        const uint32_t lt = (uint32_t)(gtile - im.tile_begin);
+        x0 = (int)(lt % (uint32_t)im.tiles_x) * RT;
+        y0 = (int)(lt / (uint32_t)im.tiles_x) * RT;
+        tw = min(RT, im.w - x0);
+        th = min(RT, im.h - y0);
+    }
     const bool hasL = x0 > 0, hasR = x0 + tw < im.w, hasU = y0 > 0, hasD = y0 + th < im.h;
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         sm.count = 0;
         sm.open = 0;
         sm.err = 0;
         sm.zflags = 0;
+        sm.nonuni = 0;
     }
-    row_tile_load(sm, im, x0, y0, tw, th);
+    if (live) row_tile_load(sm, im, x0, y0, tw, th, tid);
     __syncthreads();
 
     // ---- classification ---------------------------------------------------
@@ -379,95 +386,74 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
     // a uniform tile (one value): every valid pixel equals its right and lower
     // neighbours inside the tile, and the halves meet equal
     const bool all_eq = H == (VM & (VM >> 1)) && (!dv || D == VM) && (h == 1 || !v31 || XH);
-    const bool uniform = __syncthreads_and(all_eq) != 0;
-    uint8_t* rec = ws.rec + gtile * 4 * RT;
-    if (uniform) {
-        // one zone: OR the evidence, then everything is known
+    if (!__all(all_eq)) sm.nonuni = 1u;
+    {
+        // a uniform tile is one zone: its evidence is the OR over the tile
         const uint32_t f = (G ? F_G : 0u) | (Sm ? F_S : 0u) | (SEAM ? F_SEAM : 0u);
         const uint32_t wf = (__ballot(f & F_G) ? F_G : 0u) | (__ballot(f & F_S) ? F_S : 0u) |
                             (__ballot(f & F_SEAM) ? F_SEAM : 0u);
         if (y == 0 && wf) atomicOr(&sm.zflags, wf);
-        __syncthreads();
-        const uint32_t zf = sm.zflags;
-        if (threadIdx.x == 0 && (zf & F_SEAM)) {
-            sm.count = 1;
-            ws.sbits[gtile * S] = (uint8_t)(zf & 3u);
-        }
-        const uint32_t mxb = (zf & F_G) ? 0u : 0xFFFFFFFFu, mnb = (zf & F_S) ? 0u : 0xFFFFFFFFu;
-        if (y < th && nb > 0) {
-            const int64_t o = (int64_t)(y0 + y) * im.opitch + x0 + 32 * h;
-            if (nb == 32) {
-                store_u128(im.mx + o, make_uint4(mxb, mxb, mxb, mxb));
-                store_u128(im.mx + o + 16, make_uint4(mxb, mxb, mxb, mxb));
-                store_u128(im.mn + o, make_uint4(mnb, mnb, mnb, mnb));
-                store_u128(im.mn + o + 16, make_uint4(mnb, mnb, mnb, mnb));
-            } else {
-                for (int k = 0; k < nb; ++k) {
-                    im.mx[o + k] = (uint8_t)mxb;
-                    im.mn[o + k] = (uint8_t)mnb;
-                }
-            }
-        }
-        // records: fragment 0 everywhere on the perimeter
-        if (y < 32) {
-            rec[32 * h + y] = 0;
-            rec[RT + 32 * h + y] = 0;
-        }
-        if (h == 0) rec[2 * RT + y] = 0;
-        if (h == 1) rec[3 * RT + y] = 0;
-    } else {
-        // ---- runs: union-find over run starts ------------------------------
-        {
-            // every entry its own parent, 16 bytes (8 entries) per thread and
-            // store, consecutive threads on consecutive bytes
-            uint4* dst = reinterpret_cast<uint4*>(sm.lab);
+    }
+    {
+        // every label entry its own parent, 16 bytes (8 entries) per thread
+        // and store, consecutive threads on consecutive bytes
+        uint4* dst = reinterpret_cast<uint4*>(sm.lab);
 #pragma unroll
-            for (int q = 0; q < RT * RT / 8 / RT_THREADS; ++q) {
-                const uint32_t c = threadIdx.x + q * RT_THREADS;
-                const uint32_t k = (8 * c) * 0x00010001u + 0x00010000u;
-                dst[c] = make_uint4(k, k + 0x00020002u, k + 0x00040004u, k + 0x00060006u);
-            }
+        for (int q = 0; q < RT * RT / 8 / RT_THREADS; ++q) {
+            const uint32_t c = tid + q * RT_THREADS;
+            const uint32_t k = (8 * c) * 0x00010001u + 0x00010000u;
+            dst[c] = make_uint4(k, k + 0x00020002u, k + 0x00040004u, k + 0x00060006u);
         }
-        __syncthreads();
-        const uint32_t RSup = from_lane_above(RS);
-        {
-            const RowEdges e = row_edges(U, UL, UR, H);
-            const uint32_t nup = nrow - 1u;   // the row above
-            // A run's first edge is its initial link (Komura): a plain store of
-            // the run above as its parent.  Links always point one row up, so
-            // they form a forest; the other edges are unions on it.
-            const uint32_t first1 = e.first & ~((fill_fwd(e.first, H) & H) << 1);
-            uint32_t todo = first1;
-            while (todo) {
-                const int bb = __builtin_ctz(todo);
-                todo &= todo - 1u;
-                const int k = bb - (int)bit(e.kl, bb) + (int)bit(e.kr, bb);
-                rl_st(sm.lab, nrow + (high_at_or_below(RS, bb) << 6), nup + (high_at_or_below(RSup, k) << 6));
-            }
-            __syncthreads();
-            todo = e.first & ~first1;
-            while (todo) {
-                const int bb = __builtin_ctz(todo);
-                todo &= todo - 1u;
-                const int k = bb - (int)bit(e.kl, bb) + (int)bit(e.kr, bb);
-                rl_unite(sm.lab, nrow + (high_at_or_below(RS, bb) << 6), nup + (high_at_or_below(RSup, k) << 6));
-            }
-            todo = e.second;
-            while (todo) {
-                const int bb = __builtin_ctz(todo);
-                todo &= todo - 1u;
-                rl_unite(sm.lab, nrow + (high_at_or_below(RS, bb) << 6), nup + (high_at_or_below(RSup, bb + 1) << 6));
-            }
-            if (XH | XDR | XUR) {
-                const uint32_t a31 = nrow + (high_at_or_below(RS, 31) << 6);
-                if (XH) rl_unite(sm.lab, a31, rnode(y, 32));
-                if (XDR) rl_unite(sm.lab, a31, rnode(y + 1, 32));
-                if (XUR) rl_unite(sm.lab, a31, rnode(y - 1, 32));
-            }
+    }
+    // Every barrier below is passed by all threads of the workgroup, which
+    // may hold tiles that take different branches.
+    __syncthreads();
+    const bool uniform = sm.nonuni == 0;
+    uint8_t* rec = ws.rec + gtile * 4 * RT;
+
+    // ---- runs: union-find over run starts ----------------------------------
+    const uint32_t RSup = from_lane_above(RS);
+    const RowEdges e = row_edges(U, UL, UR, H);
+    const uint32_t nup = nrow - 1u;   // the row above
+    // A run's first edge is its initial link (Komura): a plain store of the
+    // run above as its parent.  Links always point one row up, so they form a
+    // forest; the other edges are unions on it.
+    const uint32_t first1 = e.first & ~((fill_fwd(e.first, H) & H) << 1);
+    if (!uniform) {
+        uint32_t todo = first1;
+        while (todo) {
+            const int bb = __builtin_ctz(todo);
+            todo &= todo - 1u;
+            const int k = bb - (int)bit(e.kl, bb) + (int)bit(e.kr, bb);
+            rl_st(sm.lab, nrow + (high_at_or_below(RS, bb) << 6), nup + (high_at_or_below(RSup, k) << 6));
         }
-        __syncthreads();
+    }
+    __syncthreads();
+    if (!uniform) {
+        uint32_t todo = e.first & ~first1;
+        while (todo) {
+            const int bb = __builtin_ctz(todo);
+            todo &= todo - 1u;
+            const int k = bb - (int)bit(e.kl, bb) + (int)bit(e.kr, bb);
+            rl_unite(sm.lab, nrow + (high_at_or_below(RS, bb) << 6), nup + (high_at_or_below(RSup, k) << 6));
+        }
+        todo = e.second;
+        while (todo) {
+            const int bb = __builtin_ctz(todo);
+            todo &= todo - 1u;
+            rl_unite(sm.lab, nrow + (high_at_or_below(RS, bb) << 6), nup + (high_at_or_below(RSup, bb + 1) << 6));
+        }
+        if (XH | XDR | XUR) {
+            const uint32_t a31 = nrow + (high_at_or_below(RS, 31) << 6);
+            if (XH) rl_unite(sm.lab, a31, rnode(y, 32));
+            if (XDR) rl_unite(sm.lab, a31, rnode(y + 1, 32));
+            if (XUR) rl_unite(sm.lab, a31, rnode(y - 1, 32));
+        }
+    }
+    __syncthreads();
+    uint32_t roots = 0;
+    if (!uniform) {
         // ---- evidence of each run at its root; compression -----------------
-        uint32_t roots = 0;
         {
             const uint32_t rfg = fill_bwd(G & NS, H) & RS, rfs = fill_bwd(Sm & NS, H) & RS,
                            rfx = fill_bwd(SEAM, H) & RS;
@@ -512,7 +498,11 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                                           __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
-        __syncthreads();
+    }
+    __syncthreads();
+    uint32_t outMX = 0, outMN = 0;   // this half-row's final (or provisional) masks
+    uint32_t opens = 0;              // run starts of open fragments
+    if (!uniform) {
         // ---- number the seam zones (fragments) -----------------------------
         // A root's entry becomes RL_ROOT | evidence | id; only its own lane
         // writes it, and the evidence bits stay, so the mask pass below reads
@@ -559,25 +549,23 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         }
         const uint32_t mxs = RS & ~zg, mns = RS & ~zs;
         // a fragment that already has both kinds of evidence is final
-        const uint32_t opens = zx & ~(zg & zs);
+        opens = zx & ~(zg & zs);
         if (opens) sm.open = 1u;
         const uint32_t MX = ((SGL & ~G) | fill_fwd(mxs, H)) & VM;
         const uint32_t MN = ((SGL & ~Sm) | fill_fwd(mns, H)) & VM;
-        if (y < th && nb > 0) {
-            const int64_t o = (int64_t)(y0 + y) * im.opitch + x0 + 32 * h;
-            if (nb == 32) {
-                store_u128(im.mx + o, make_uint4(mask_bytes(MX, 0), mask_bytes(MX, 1), mask_bytes(MX, 2), mask_bytes(MX, 3)));
-                store_u128(im.mx + o + 16, make_uint4(mask_bytes(MX, 4), mask_bytes(MX, 5), mask_bytes(MX, 6), mask_bytes(MX, 7)));
-                store_u128(im.mn + o, make_uint4(mask_bytes(MN, 0), mask_bytes(MN, 1), mask_bytes(MN, 2), mask_bytes(MN, 3)));
-                store_u128(im.mn + o + 16, make_uint4(mask_bytes(MN, 4), mask_bytes(MN, 5), mask_bytes(MN, 6), mask_bytes(MN, 7)));
-            } else {
-                for (int k = 0; k < nb; ++k) {
-                    im.mx[o + k] = (uint8_t)(bit(MX, k) * 0xFFu);
-                    im.mn[o + k] = (uint8_t)(bit(MN, k) * 0xFFu);
-                }
-            }
+        outMX = MX;
+        outMN = MN;
+    } else {
+        const uint32_t zf = sm.zflags;
+        if (tid == 0 && (zf & F_SEAM)) {
+            sm.count = 1;
+            ws.sbits[gtile * S] = (uint8_t)(zf & 3u);
         }
-        __syncthreads();   // every root numbered
+        outMX = (zf & F_G) ? 0u : VM;
+        outMN = (zf & F_S) ? 0u : VM;
+    }
+    __syncthreads();   // every root numbered
+    if (!uniform) {
         // ---- perimeter records -------------------------------------------------
         // A seam pixel's zone must have a fragment id (invariant code 2).
         {
@@ -612,8 +600,8 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
             uint32_t cw[8];
 #pragma unroll
             for (int d = 0; d < 8; ++d)
-                cw[d] = REC_CLOSED * 0x01010101u | spread4((MX >> (4 * d)) & 0xFu) |
-                        spread4((MN >> (4 * d)) & 0xFu) << 1;
+                cw[d] = REC_CLOSED * 0x01010101u | spread4((outMX >> (4 * d)) & 0xFu) |
+                        spread4((outMN >> (4 * d)) & 0xFu) << 1;
             *reinterpret_cast<uint4*>(row) = make_uint4(cw[0], cw[1], cw[2], cw[3]);
             *reinterpret_cast<uint4*>(row + 16) = make_uint4(cw[4], cw[5], cw[6], cw[7]);
             uint32_t m = opens;
@@ -635,8 +623,20 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                 }
             }
         }
+    } else if (live) {
+        // records: fragment 0 everywhere on the perimeter
+        if (y < 32) {
+            rec[32 * h + y] = 0;
+            rec[RT + 32 * h + y] = 0;
+        }
+        if (h == 0) rec[2 * RT + y] = 0;
+        if (h == 1) rec[3 * RT + y] = 0;
     }
     __syncthreads();
+    // The masks leave through LDS as bits (the labels are no longer read), so
+    // that each store covers whole 64-byte tile rows: a lane's own row would
+    // make every store instruction touch 64 rows.
+    reinterpret_cast<uint2*>(sm.lab)[tid] = make_uint2(outMX, outMN);
     const uint32_t count = sm.count;
     const bool open = !uniform && sm.open != 0;
     if (open) {
@@ -644,7 +644,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         uint4* dst = reinterpret_cast<uint4*>(plane);
 #pragma unroll
         for (int i = 0; i < RT * RT / 16 / RT_THREADS; ++i) {
-            const int c = threadIdx.x + i * RT_THREADS;   // 16-byte chunk: row c / 4, part c % 4
+            const int c = tid + i * RT_THREADS;   // 16-byte chunk: row c / 4, part c % 4
             dst[c] = *reinterpret_cast<const uint4*>(sm.plane + (c >> 2) * RT_P + 16 * (c & 3));
         }
     }
@@ -653,9 +653,28 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         if (h == 1) eq[y] = (uint8_t)eqr;
         if (y < 32) eq[RT + 32 * h + y] = (uint8_t)eqb;
     }
-    if (threadIdx.x == 0)
+    if (tid == 0 && live)
         ws.counts[gtile] = min(count, (uint32_t)S) | (uniform ? CNT_UNIFORM : 0u) | (open ? CNT_OPEN : 0u) |
                            (sm.err << COUNT_ERR_SHIFT);
+    __syncthreads();
+    // 16-pixel chunk c: tile row c / 4, columns 16 (c % 4) .. + 15
+#pragma unroll
+    for (int i = 0; i < RT * RT / 16 / RT_THREADS; ++i) {
+        const int c = tid + i * RT_THREADS, r = c >> 2, q = c & 3;
+        if (r >= th || 16 * q >= tw) continue;
+        const uint2 bm = reinterpret_cast<const uint2*>(sm.lab)[(q >> 1) * 64 + r];
+        const uint32_t mx16 = bm.x >> (16 * (q & 1)), mn16 = bm.y >> (16 * (q & 1));
+        const int64_t o = (int64_t)(y0 + r) * im.opitch + x0 + 16 * q;
+        if (16 * q + 16 <= tw) {
+            store_u128(im.mx + o, make_uint4(mask_bytes(mx16, 0), mask_bytes(mx16, 1), mask_bytes(mx16, 2), mask_bytes(mx16, 3)));
+            store_u128(im.mn + o, make_uint4(mask_bytes(mn16, 0), mask_bytes(mn16, 1), mask_bytes(mn16, 2), mask_bytes(mn16, 3)));
+        } else {
+            for (int k = 0; 16 * q + k < tw; ++k) {
+                im.mx[o + k] = (uint8_t)(bit(mx16, k) * 0xFFu);
+                im.mn[o + k] = (uint8_t)(bit(mn16, k) * 0xFFu);
+            }
+        }
+    }
 }
 
 }  // namespace ssx
