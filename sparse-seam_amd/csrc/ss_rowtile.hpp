@@ -42,18 +42,25 @@ constexpr int RT_NTW = 1;         // tiles per workgroup
 constexpr int RT_P = 80;          // staged row pitch: tile column c of staged row r at r * RT_P + 16 + c
 constexpr int RT_ROWS = 66;       // staged rows: tile rows -1 .. 64
 
-// Union-find nodes are indexed column-major, id = 64 x + y for pixel (y, x),
+// Union-find nodes are indexed column-major, index 64 x + y for pixel (y, x),
 // so the entries the 64 lanes (rows) touch at one column are consecutive
 // halfwords, not one per 128-byte row (which all fall in the same LDS bank).
-__device__ __forceinline__ uint32_t rnode(int y, int x) { return (uint32_t)((x << 6) | y); }
+// A node is named by its entry's byte offset in the label array (twice its
+// index), and parents are stored in that form: a loaded parent is the next
+// address as it is.
+__device__ __forceinline__ uint32_t rnode(int y, int x) { return (uint32_t)((x << 7) | (y << 1)); }
+constexpr int RL_COL = 7;       // rnode(y, x + 1) = rnode(y, x) + (1 << RL_COL)
+constexpr uint32_t RL_ROW = 2;  // rnode(y + 1, x) = rnode(y, x) + RL_ROW
 
-// Label entries (16 bits): parent (bits 0-11) while the unions run; evidence
-// bits 12-14 OR'ed at roots; once numbered, a root holds RL_ROOT | evidence |
-// fragment id (RL_NOID if it is not a seam zone).
-constexpr uint32_t RL_IDX = 0x0FFFu;
-constexpr int RL_FSHIFT = 12;
-constexpr uint32_t RL_ROOT = 0x8000u;
+// Label entries (16 bits): the parent's offset (bits 1-12; bit 0 clear) while
+// the unions run; evidence bits 13-15 OR'ed at roots; once numbered, a root
+// holds RL_ROOT | fragment id << 1 | evidence (id RL_NOID if it is not a seam
+// zone).
+constexpr uint32_t RL_IDX = 0x1FFEu;
+constexpr int RL_FSHIFT = 13;
+constexpr uint32_t RL_ROOT = 1u;
 constexpr uint32_t RL_NOID = 0x0FFFu;
+__device__ __forceinline__ uint32_t rl_id(uint32_t e) { return (e >> 1) & 0xFFFu; }
 
 struct RowTileSmem {
     union {
@@ -64,15 +71,19 @@ struct RowTileSmem {
     uint32_t count, open, err, zflags, nonuni;
 };
 
-__device__ __forceinline__ uint32_t rl_ld(const uint16_t* lab, uint32_t p) {
-    return __hip_atomic_load(lab + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+__device__ __forceinline__ uint32_t rl_ld(const uint16_t* lab, uint32_t o) {
+    return __hip_atomic_load(reinterpret_cast<const uint16_t*>(reinterpret_cast<const uint8_t*>(lab) + o),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
-__device__ __forceinline__ void rl_st(uint16_t* lab, uint32_t p, uint32_t v) {
-    __hip_atomic_store(lab + p, (uint16_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+__device__ __forceinline__ void rl_st(uint16_t* lab, uint32_t o, uint32_t v) {
+    __hip_atomic_store(reinterpret_cast<uint16_t*>(reinterpret_cast<uint8_t*>(lab) + o), (uint16_t)v,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
-__device__ __forceinline__ uint32_t* rl_word(uint16_t* lab, uint32_t p) {
-    return reinterpret_cast<uint32_t*>(lab) + (p >> 1);
+// The 32-bit word holding entry o, and the entry's bit position in it.
+__device__ __forceinline__ uint32_t* rl_word(uint16_t* lab, uint32_t o) {
+    return reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(lab) + (o & ~3u));
 }
+__device__ __forceinline__ int rl_shift(uint32_t o) { return (int)(o & 2u) * 8; }
 
 // Find with path halving (unions only: entries hold plain parents).
 __device__ __forceinline__ uint32_t rl_find_h(uint16_t* lab, uint32_t x) {
@@ -99,7 +110,7 @@ __device__ __forceinline__ void rl_unite(uint16_t* lab, uint32_t a, uint32_t b) 
         if (a == b) return;
         const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
         uint32_t* w = rl_word(lab, hi);
-        const int sh = (int)(hi & 1u) * 16;
+        const int sh = rl_shift(hi);
         uint32_t old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         for (;;) {
             const uint32_t cur = (old >> sh) & 0xFFFFu;
@@ -111,21 +122,6 @@ __device__ __forceinline__ void rl_unite(uint16_t* lab, uint32_t a, uint32_t b) 
         }
         a = (old >> sh) & RL_IDX;
         b = lo;
-    }
-}
-
-// Find with path halving once the roots carry evidence bits.  The unions
-// leave tall zones as chains of runs (each row linked to the one above), and
-// all lanes halve their walks at once, so a chain's depth halves per step.
-__device__ __forceinline__ uint32_t rl_find_hm(uint16_t* lab, uint32_t x) {
-    uint32_t p = rl_ld(lab, x) & RL_IDX;
-    if (p == x) return x;
-    for (;;) {
-        const uint32_t gp = rl_ld(lab, p) & RL_IDX;
-        if (gp == p) return p;
-        rl_st(lab, x, gp);   // x is not a root: its entry has no evidence bits
-        x = gp;
-        p = rl_ld(lab, x) & RL_IDX;
     }
 }
 
@@ -365,7 +361,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
     const uint32_t NS = VM & (H | (H << 1) | U | UL | UR | D | DL | DR | cross | SEAM);
     const uint32_t RS = NS & ~(H << 1);
     const uint32_t SGL = VM & ~NS;
-    const uint32_t nrow = rnode(y, 32 * h);   // node of column 32h of this row; column 32h + b is nrow + 64 b
+    const uint32_t nrow = rnode(y, 32 * h);   // node of column 32h of this row; column 32h + b: nrow + (b << RL_COL)
     // cross-seam equality bits (right column / bottom row)
     uint32_t eqr = 0;
     if (h == 1 && tw == 64 && hasR && y < th) {
@@ -401,8 +397,9 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
 #pragma unroll
         for (int q = 0; q < RT * RT / 8 / RT_THREADS; ++q) {
             const uint32_t c = tid + q * RT_THREADS;
-            const uint32_t k = (8 * c) * 0x00010001u + 0x00010000u;
-            dst[c] = make_uint4(k, k + 0x00020002u, k + 0x00040004u, k + 0x00060006u);
+            // entry i holds its own offset 2i
+            const uint32_t k = (16 * c) * 0x00010001u + 0x00020000u;
+            dst[c] = make_uint4(k, k + 0x00040004u, k + 0x00080008u, k + 0x000C000Cu);
         }
     }
     // Every barrier below is passed by all threads of the workgroup, which
@@ -414,7 +411,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
     // ---- runs: union-find over run starts ----------------------------------
     const uint32_t RSup = from_lane_above(RS);
     const RowEdges e = row_edges(U, UL, UR, H);
-    const uint32_t nup = nrow - 1u;   // the row above
+    const uint32_t nup = nrow - RL_ROW;   // the row above
     // A run's first edge is its initial link (Komura): a plain store of the
     // run above as its parent.  Links always point one row up, so they form a
     // forest; the other edges are unions on it.
@@ -425,7 +422,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
             const int bb = __builtin_ctz(todo);
             todo &= todo - 1u;
             const int k = bb - (int)bit(e.kl, bb) + (int)bit(e.kr, bb);
-            rl_st(sm.lab, nrow + (high_at_or_below(RS, bb) << 6), nup + (high_at_or_below(RSup, k) << 6));
+            rl_st(sm.lab, nrow + (high_at_or_below(RS, bb) << RL_COL), nup + (high_at_or_below(RSup, k) << RL_COL));
         }
     }
     __syncthreads();
@@ -435,16 +432,16 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
             const int bb = __builtin_ctz(todo);
             todo &= todo - 1u;
             const int k = bb - (int)bit(e.kl, bb) + (int)bit(e.kr, bb);
-            rl_unite(sm.lab, nrow + (high_at_or_below(RS, bb) << 6), nup + (high_at_or_below(RSup, k) << 6));
+            rl_unite(sm.lab, nrow + (high_at_or_below(RS, bb) << RL_COL), nup + (high_at_or_below(RSup, k) << RL_COL));
         }
         todo = e.second;
         while (todo) {
             const int bb = __builtin_ctz(todo);
             todo &= todo - 1u;
-            rl_unite(sm.lab, nrow + (high_at_or_below(RS, bb) << 6), nup + (high_at_or_below(RSup, bb + 1) << 6));
+            rl_unite(sm.lab, nrow + (high_at_or_below(RS, bb) << RL_COL), nup + (high_at_or_below(RSup, bb + 1) << RL_COL));
         }
         if (XH | XDR | XUR) {
-            const uint32_t a31 = nrow + (high_at_or_below(RS, 31) << 6);
+            const uint32_t a31 = nrow + (high_at_or_below(RS, 31) << RL_COL);
             if (XH) rl_unite(sm.lab, a31, rnode(y, 32));
             if (XDR) rl_unite(sm.lab, a31, rnode(y + 1, 32));
             if (XUR) rl_unite(sm.lab, a31, rnode(y - 1, 32));
@@ -466,7 +463,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                 m &= m - 1u;
                 const int b2 = m ? __builtin_ctz(m) : b1;
                 m &= m - 1u;
-                const uint32_t p1 = nrow + (b1 << 6), p2 = nrow + (b2 << 6);
+                const uint32_t p1 = nrow + (b1 << RL_COL), p2 = nrow + (b2 << RL_COL);
                 // find with path halving, one exit: at a root the grandparent
                 // is the parent itself
                 uint32_t x1 = p1, x2 = p2;
@@ -491,10 +488,10 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                 const uint32_t f2 = bit(rfg, b2) | bit(rfs, b2) << 1 | bit(rfx, b2) << 2;
                 const uint32_t h1 = rl_ld(sm.lab, x1) >> RL_FSHIFT, h2 = rl_ld(sm.lab, x2) >> RL_FSHIFT;
                 if (f1 & ~h1)
-                    __hip_atomic_fetch_or(rl_word(sm.lab, x1), f1 << (RL_FSHIFT + 16 * (x1 & 1u)), __ATOMIC_RELAXED,
+                    __hip_atomic_fetch_or(rl_word(sm.lab, x1), f1 << (RL_FSHIFT + rl_shift(x1)), __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (f2 & ~h2)
-                    __hip_atomic_fetch_or(rl_word(sm.lab, x2), f2 << (RL_FSHIFT + 16 * (x2 & 1u)), __ATOMIC_RELAXED,
+                    __hip_atomic_fetch_or(rl_word(sm.lab, x2), f2 << (RL_FSHIFT + rl_shift(x2)), __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
@@ -512,7 +509,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
             while (m) {
                 const int bb = __builtin_ctz(m);
                 m &= m - 1u;
-                const uint32_t p = nrow + (bb << 6);
+                const uint32_t p = nrow + (bb << RL_COL);
                 const uint32_t f = (rl_ld(sm.lab, p) >> RL_FSHIFT) & 7u;
                 uint32_t id = RL_NOID;
                 if (f & F_SEAM) {
@@ -524,7 +521,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                         ws.sbits[gtile * S + j] = (uint8_t)(f & 3u);
                     }
                 }
-                rl_st(sm.lab, p, RL_ROOT | f << RL_FSHIFT | id);
+                rl_st(sm.lab, p, RL_ROOT | id << 1 | f << RL_FSHIFT);
             }
         }
         // ---- masks: each run takes its zone's evidence ----------------------
@@ -536,7 +533,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                 m &= m - 1u;
                 const int b2 = m ? __builtin_ctz(m) : b1;
                 m &= m - 1u;
-                const uint32_t p1 = nrow + (b1 << 6), p2 = nrow + (b2 << 6);
+                const uint32_t p1 = nrow + (b1 << RL_COL), p2 = nrow + (b2 << RL_COL);
                 const uint32_t e1 = rl_ld(sm.lab, p1), e2 = rl_ld(sm.lab, p2);
                 // the roots' entries (read unconditionally: an index below 4096 either way)
                 const uint32_t r1 = rl_ld(sm.lab, e1 & RL_IDX), r2 = rl_ld(sm.lab, e2 & RL_IDX);
@@ -578,20 +575,20 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
             const uint32_t rsm = y < 32 ? rs_r : rs_b, nsm = y < 32 ? ns_r : ns_b, sxm = y < 32 ? sx_r : sx_b;
             if (x < tw && bit(nsm, bl)) {
                 const uint32_t e = rl_zone(sm.lab, rnode(ry, 32 * h + high_at_or_below(rsm, bl)));
-                rec[(y < 32 ? 0 : RT) + x] = (uint8_t)(e & 0xFFu);
-                if (bit(sxm, bl) && (e & RL_IDX) == RL_NOID) sm.err = 2u;
+                rec[(y < 32 ? 0 : RT) + x] = (uint8_t)rl_id(e);
+                if (bit(sxm, bl) && rl_id(e) == RL_NOID) sm.err = 2u;
             }
             // left column (wave 0) and right column (the wave holding column tw - 1)
             if (h == 0 && (NS & 1u)) {
                 const uint32_t e = rl_zone(sm.lab, nrow);
-                rec[2 * RT + y] = (uint8_t)(e & 0xFFu);
-                if ((SEAM & 1u) && (e & RL_IDX) == RL_NOID) sm.err = 2u;
+                rec[2 * RT + y] = (uint8_t)rl_id(e);
+                if ((SEAM & 1u) && rl_id(e) == RL_NOID) sm.err = 2u;
             }
             const int br = tw - 1 - 32 * h;
             if (br >= 0 && br < 32 && bit(NS, br)) {
-                const uint32_t e = rl_zone(sm.lab, nrow + (high_at_or_below(RS, br) << 6));
-                rec[3 * RT + y] = (uint8_t)(e & 0xFFu);
-                if (bit(SEAM, br) && (e & RL_IDX) == RL_NOID) sm.err = 2u;
+                const uint32_t e = rl_zone(sm.lab, nrow + (high_at_or_below(RS, br) << RL_COL));
+                rec[3 * RT + y] = (uint8_t)rl_id(e);
+                if (bit(SEAM, br) && rl_id(e) == RL_NOID) sm.err = 2u;
             }
         }
         // ---- record plane (tiles with an open fragment) ----------------------
@@ -608,7 +605,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
             while (m) {
                 const int bb = __builtin_ctz(m);
                 m &= m - 1u;
-                const uint32_t id = rl_zone(sm.lab, nrow + (bb << 6)) & 0xFFu;
+                const uint32_t id = rl_id(rl_zone(sm.lab, nrow + (bb << RL_COL))) & 0xFFu;
                 // the run: bb .. the first column without an equal right link
                 const int be = __builtin_ctz(~H & (0xFFFFFFFFu << bb));
                 int q = bb;
