@@ -73,4 +73,27 @@ __host__ __device__ __forceinline__ RowEdges row_edges(uint32_t u, uint32_t ul, 
     return e;
 }
 
+// The relations across columns 31 | 32 of a row y: XH (y, 31)-(y, 32), XDR
+// (y, 31)-(y + 1, 32), XUR (y, 31)-(y - 1, 32).  They are equalities of pixel
+// values, so two of them through a common pixel imply the third; one that
+// follows from kept relations and from the vertical relations inside each
+// half is dropped:
+//   XH(y)  when XH(y - 1) and (y - 1, 31)-(y, 31): then (y - 1, 32)-(y, 32);
+//   XDR(y) when (y, 31)-(y + 1, 31): then XH(y + 1); or when XH(y): then
+//          (y, 32)-(y + 1, 32);
+//   XUR(y) when (y - 1, 31)-(y, 31): then XH(y - 1); or when XH(y): then
+//          (y - 1, 32)-(y, 32).
+// Only XH relations justify a dropped one, and an XH only through the row
+// above, so the kept set implies every relation.  Inputs are 0 / 1: xh, xdr,
+// xur of row y; xh_up: XH of row y - 1; d31, d31_up: column 31 equal between
+// rows y and y + 1, and between y - 1 and y.  Returns bit 0 = keep XH, bit 1
+// = keep XDR, bit 2 = keep XUR.
+__host__ __device__ __forceinline__ uint32_t cross_keep(uint32_t xh, uint32_t xdr, uint32_t xur, uint32_t xh_up,
+                                                        uint32_t d31, uint32_t d31_up) {
+    const uint32_t kh = xh & ~(xh_up & d31_up);
+    const uint32_t kdr = xdr & ~d31 & ~xh;
+    const uint32_t kur = xur & ~d31_up & ~xh;
+    return (kh & 1u) | (kdr & 1u) << 1 | (kur & 1u) << 2;
+}
+
 }  // namespace ssx

@@ -68,7 +68,7 @@ struct RowTileSmem {
         alignas(16) uint8_t plane[RT * RT_P];           // record plane (row pitch RT_P), built after the staged rows are read
     };
     alignas(16) uint16_t lab[RT * RT];
-    uint32_t count, open, err, zflags, nonuni;
+    uint32_t count, open, err, zflags, nonuni, anygs;
 };
 
 __device__ __forceinline__ uint32_t rl_ld(const uint16_t* lab, uint32_t o) {
@@ -98,17 +98,36 @@ __device__ __forceinline__ uint32_t rl_find_h(uint16_t* lab, uint32_t x) {
     }
 }
 
-// Union of two trees: the root with the larger index is linked under the
+// Run starts whose run holds both a strictly greater and a strictly smaller
+// neighbour ("GS runs") are marked per row and half in a 32-bit word of an
+// LDS array (owngs[2 y + h], bit b = column 32 h + b).  A node's key orders
+// GS runs before the others, then by row, then by column:
+//   key(o) = (GS ? 0 : 1) << 12 | y << 6 | x.
+// Every link goes from a larger key to a smaller one, so the root of a tree
+// holds the smallest key in it: a zone's root is a GS run whenever the zone
+// has one, and such a zone has both kinds of evidence.
+__device__ __forceinline__ uint32_t rl_gs(const uint32_t* owngs, uint32_t o) {
+    const uint32_t y = (o >> 1) & 63u, x = o >> 7;
+    return (owngs[2 * y + (x >> 5)] >> (x & 31u)) & 1u;
+}
+__device__ __forceinline__ uint32_t rl_key(const uint32_t* owngs, bool anygs, uint32_t o) {
+    // without GS runs in the tile every key has the same top bit
+    const uint32_t rm = (((o >> 1) & 63u) << 6) | (o >> 7);
+    return anygs ? ((rl_gs(owngs, o) ^ 1u) << 12) | rm : rm;
+}
+
+// Union of two trees: the root with the larger key is linked under the
 // other by a CAS on the 32-bit word holding its entry (the other half may
 // change meanwhile: then the CAS is retried as long as the entry is still a
 // root); a root that got a parent meanwhile is walked again from that parent.
 // Every link joins two trees, so the parents stay a forest.
-__device__ __forceinline__ void rl_unite(uint16_t* lab, uint32_t a, uint32_t b) {
+__device__ __forceinline__ void rl_unite(uint16_t* lab, const uint32_t* owngs, bool anygs, uint32_t a, uint32_t b) {
     for (;;) {
         a = rl_find_h(lab, a);
         b = rl_find_h(lab, b);
         if (a == b) return;
-        const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+        const bool ahi = rl_key(owngs, anygs, a) > rl_key(owngs, anygs, b);
+        const uint32_t hi = ahi ? a : b, lo = ahi ? b : a;
         uint32_t* w = rl_word(lab, hi);
         const int sh = rl_shift(hi);
         uint32_t old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -125,11 +144,14 @@ __device__ __forceinline__ void rl_unite(uint16_t* lab, uint32_t a, uint32_t b) 
     }
 }
 
-// Zone entry of a run start once the roots are numbered.
+// Zone entry of a run start whose path is compressed (its entry is its root
+// or names it), and the fragment id it carries: a root that was never
+// numbered is not a seam zone.
 __device__ __forceinline__ uint32_t rl_zone(const uint16_t* lab, uint32_t p) {
     const uint32_t e = rl_ld(lab, p);
-    return (e & RL_ROOT) ? e : rl_ld(lab, e & RL_IDX);
+    return ((e & RL_ROOT) || (e & RL_IDX) == p) ? e : rl_ld(lab, e & RL_IDX);
 }
+__device__ __forceinline__ uint32_t rl_zone_id(uint32_t z) { return (z & RL_ROOT) ? rl_id(z) : RL_NOID; }
 
 // Lane y receives lane y - 1's value (lane 0: 0).
 __device__ __forceinline__ uint32_t from_lane_above(uint32_t v) {
@@ -275,6 +297,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         sm.err = 0;
         sm.zflags = 0;
         sm.nonuni = 0;
+        sm.anygs = 0;
     }
     if (live) row_tile_load(sm, im, x0, y0, tw, th, tid);
     __syncthreads();
@@ -410,14 +433,28 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
 
     // ---- runs: union-find over run starts ----------------------------------
     const uint32_t RSup = from_lane_above(RS);
+    const uint32_t xkeep = cross_keep(XH, XDR, XUR, from_lane_above(XH), bit(D, 31), bit(U, 31));
     const RowEdges e = row_edges(U, UL, UR, H);
     const uint32_t nup = nrow - RL_ROW;   // the row above
+    // each run's evidence at its start: greater, smaller and seam bits
+    const uint32_t rfg = fill_bwd(G & NS, H) & RS, rfs = fill_bwd(Sm & NS, H) & RS, rfx = fill_bwd(SEAM, H) & RS;
+    const uint32_t GSR = rfg & rfs;   // GS runs (both kinds of evidence)
     // A run's first edge is its initial link (Komura): a plain store of the
     // run above as its parent.  Links always point one row up, so they form a
-    // forest; the other edges are unions on it.
+    // forest; the other edges are unions on it.  A plain link goes to a
+    // smaller key (rl_key) unless a GS run would link under a run that is not
+    // one: such a first edge is a union instead.
     const uint32_t first1 = e.first & ~((fill_fwd(e.first, H) & H) << 1);
+    const uint32_t NGSup = from_lane_above(fill_fwd(RS & ~GSR, H));   // pixels of non-GS runs above
+    const uint32_t viol = first1 & fill_fwd(GSR, H) &
+                          ((NGSup & ~e.kl & ~e.kr) | ((NGSup << 1) & e.kl) | ((NGSup >> 1) & e.kr));
+    const uint32_t link1 = first1 & ~viol;
+    // the staged rows are no longer read: their first 512 bytes hold the GS run starts
+    uint32_t* owngs = reinterpret_cast<uint32_t*>(sm.img);
     if (!uniform) {
-        uint32_t todo = first1;
+        owngs[2 * y + h] = GSR;
+        if (__ballot(GSR != 0u) && y == 0) sm.anygs = 1u;
+        uint32_t todo = link1;
         while (todo) {
             const int bb = __builtin_ctz(todo);
             todo &= todo - 1u;
@@ -426,74 +463,68 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         }
     }
     __syncthreads();
+    const bool anygs = sm.anygs != 0u;
     if (!uniform) {
-        uint32_t todo = e.first & ~first1;
-        while (todo) {
-            const int bb = __builtin_ctz(todo);
-            todo &= todo - 1u;
-            const int k = bb - (int)bit(e.kl, bb) + (int)bit(e.kr, bb);
-            rl_unite(sm.lab, nrow + (high_at_or_below(RS, bb) << RL_COL), nup + (high_at_or_below(RSup, k) << RL_COL));
-        }
-        todo = e.second;
-        while (todo) {
-            const int bb = __builtin_ctz(todo);
-            todo &= todo - 1u;
-            rl_unite(sm.lab, nrow + (high_at_or_below(RS, bb) << RL_COL), nup + (high_at_or_below(RSup, bb + 1) << RL_COL));
-        }
-        if (XH | XDR | XUR) {
-            const uint32_t a31 = nrow + (high_at_or_below(RS, 31) << RL_COL);
-            if (XH) rl_unite(sm.lab, a31, rnode(y, 32));
-            if (XDR) rl_unite(sm.lab, a31, rnode(y + 1, 32));
-            if (XUR) rl_unite(sm.lab, a31, rnode(y - 1, 32));
+        // One loop over every union of the lane: the first edges without a
+        // plain link, the second edges, then (wave 0) the cross-half
+        // relations that cross_keep keeps, at the run through column 31.
+        uint32_t tf = e.first & ~link1, ts = e.second, tx = xkeep;
+        while (tf | ts | tx) {
+            const bool isf = tf != 0u, iss = !isf && ts != 0u, isx = !isf && !iss;
+            const uint32_t cur = isf ? tf : ts;
+            const int bb = isx ? 31 : __builtin_ctz(cur);
+            const int k = isf ? bb - (int)bit(e.kl, bb) + (int)bit(e.kr, bb) : bb + 1;
+            const int j = __builtin_ctz(tx | 8u);   // 0: XH, 1: XDR, 2: XUR
+            const uint32_t nb = isx ? rnode(y, 32) + (j == 1 ? RL_ROW : 0u) - (j == 2 ? RL_ROW : 0u)
+                                    : nup + (high_at_or_below(RSup, k) << RL_COL);
+            tf = isf ? tf & (tf - 1u) : tf;
+            ts = iss ? ts & (ts - 1u) : ts;
+            tx = isx ? tx & (tx - 1u) : tx;
+            rl_unite(sm.lab, owngs, anygs, nrow + (high_at_or_below(RS, bb) << RL_COL), nb);
         }
     }
     __syncthreads();
-    uint32_t roots = 0;
     if (!uniform) {
-        // ---- evidence of each run at its root; compression -----------------
-        {
-            const uint32_t rfg = fill_bwd(G & NS, H) & RS, rfs = fill_bwd(Sm & NS, H) & RS,
-                           rfx = fill_bwd(SEAM, H) & RS;
-            // two runs per iteration (the second repeats the first when the row
-            // has none left), their two finds walked together: the dependent
-            // LDS loads of one overlap those of the other
-            uint32_t m = RS;
-            while (m) {
-                const int b1 = __builtin_ctz(m);
-                m &= m - 1u;
-                const int b2 = m ? __builtin_ctz(m) : b1;
-                m &= m - 1u;
-                const uint32_t p1 = nrow + (b1 << RL_COL), p2 = nrow + (b2 << RL_COL);
-                // find with path halving, one exit: at a root the grandparent
-                // is the parent itself
-                uint32_t x1 = p1, x2 = p2;
-                for (;;) {
-                    const uint32_t q1 = rl_ld(sm.lab, x1) & RL_IDX, q2 = rl_ld(sm.lab, x2) & RL_IDX;
-                    const uint32_t g1 = rl_ld(sm.lab, q1) & RL_IDX, g2 = rl_ld(sm.lab, q2) & RL_IDX;
-                    const bool d1 = g1 == q1, d2 = g2 == q2;
-                    if (d1 && d2) {
-                        x1 = q1;
-                        x2 = q2;
-                        break;
-                    }
-                    // a node that is not a root has no evidence bits
-                    if (!d1) { rl_st(sm.lab, x1, g1); x1 = g1; }
-                    if (!d2) { rl_st(sm.lab, x2, g2); x2 = g2; }
+        // ---- evidence of the other runs at their roots; compression ---------
+        // A GS run's zone is rooted at a GS run, which tells its evidence, so
+        // only the runs that are not GS runs report theirs.  Two runs per
+        // iteration (the second repeats the first when the row has none
+        // left), their two finds walked together: the dependent LDS loads of
+        // one overlap those of the other.
+        uint32_t m = RS & ~GSR;
+        while (m) {
+            const int b1 = __builtin_ctz(m);
+            m &= m - 1u;
+            const int b2 = m ? __builtin_ctz(m) : b1;
+            m &= m - 1u;
+            const uint32_t p1 = nrow + (b1 << RL_COL), p2 = nrow + (b2 << RL_COL);
+            // find with path halving, one exit: at a root the grandparent
+            // is the parent itself
+            uint32_t x1 = p1, x2 = p2;
+            for (;;) {
+                const uint32_t q1 = rl_ld(sm.lab, x1) & RL_IDX, q2 = rl_ld(sm.lab, x2) & RL_IDX;
+                const uint32_t g1 = rl_ld(sm.lab, q1) & RL_IDX, g2 = rl_ld(sm.lab, q2) & RL_IDX;
+                const bool d1 = g1 == q1, d2 = g2 == q2;
+                if (d1 && d2) {
+                    x1 = q1;
+                    x2 = q2;
+                    break;
                 }
-                if (x1 == p1) roots |= 1u << b1;
-                else rl_st(sm.lab, p1, x1);
-                if (x2 == p2) roots |= 1u << b2;
-                else rl_st(sm.lab, p2, x2);
-                const uint32_t f1 = bit(rfg, b1) | bit(rfs, b1) << 1 | bit(rfx, b1) << 2;
-                const uint32_t f2 = bit(rfg, b2) | bit(rfs, b2) << 1 | bit(rfx, b2) << 2;
-                const uint32_t h1 = rl_ld(sm.lab, x1) >> RL_FSHIFT, h2 = rl_ld(sm.lab, x2) >> RL_FSHIFT;
-                if (f1 & ~h1)
-                    __hip_atomic_fetch_or(rl_word(sm.lab, x1), f1 << (RL_FSHIFT + rl_shift(x1)), __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (f2 & ~h2)
-                    __hip_atomic_fetch_or(rl_word(sm.lab, x2), f2 << (RL_FSHIFT + rl_shift(x2)), __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                // a node that is not a root has no evidence bits
+                if (!d1) { rl_st(sm.lab, x1, g1); x1 = g1; }
+                if (!d2) { rl_st(sm.lab, x2, g2); x2 = g2; }
             }
+            if (x1 != p1) rl_st(sm.lab, p1, x1);
+            if (x2 != p2) rl_st(sm.lab, p2, x2);
+            const uint32_t f1 = bit(rfg, b1) | bit(rfs, b1) << 1 | bit(rfx, b1) << 2;
+            const uint32_t f2 = bit(rfg, b2) | bit(rfs, b2) << 1 | bit(rfx, b2) << 2;
+            const uint32_t h1 = rl_ld(sm.lab, x1) >> RL_FSHIFT, h2 = rl_ld(sm.lab, x2) >> RL_FSHIFT;
+            if (f1 & ~h1)
+                __hip_atomic_fetch_or(rl_word(sm.lab, x1), f1 << (RL_FSHIFT + rl_shift(x1)), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (f2 & ~h2)
+                __hip_atomic_fetch_or(rl_word(sm.lab, x2), f2 << (RL_FSHIFT + rl_shift(x2)), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
     __syncthreads();
@@ -501,18 +532,45 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
     uint32_t opens = 0;              // run starts of open fragments
     if (!uniform) {
         // ---- number the seam zones (fragments) -----------------------------
-        // A root's entry becomes RL_ROOT | evidence | id; only its own lane
-        // writes it, and the evidence bits stay, so the mask pass below reads
-        // the evidence of every zone whether its root is numbered yet or not.
+        // A run holding a seam pixel starts in the top or bottom row, or holds
+        // column 0 or column 63.  Lane b of each half takes the run start at
+        // column 32 h + b of the top row (lanes 0-31) or the bottom row (lanes
+        // 32-63); a lane of any other row takes its one seam run (its left or
+        // right column).  Each finds its root (compressing its own entry to
+        // it) and claims it: the first claim sets RL_ROOT and numbers the
+        // root; its entry becomes RL_ROOT | id << 1 | evidence.  Other lanes
+        // meanwhile read only the evidence bits, which stay.
         {
-            uint32_t m = roots;
-            while (m) {
-                const int bb = __builtin_ctz(m);
-                m &= m - 1u;
-                const uint32_t p = nrow + (bb << RL_COL);
-                const uint32_t f = (rl_ld(sm.lab, p) >> RL_FSHIFT) & 7u;
-                uint32_t id = RL_NOID;
-                if (f & F_SEAM) {
+            const int bl = y & 31;
+            const int ry = y < 32 ? 0 : th - 1;
+            const uint32_t xr = __builtin_amdgcn_readlane(rfx, 0), xb = __builtin_amdgcn_readlane(rfx, th - 1);
+            const uint32_t ctb = bit(y < 32 ? xr : xb, bl) ? rnode(ry, 32 * h + bl) : 1u;
+            const uint32_t cmid = (y > 0 && y < th - 1 && rfx) ? nrow + (__builtin_ctz(rfx) << RL_COL) : 1u;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const uint32_t pc = c == 0 ? ctb : cmid;
+                if (pc & 1u) continue;
+                uint32_t x = pc;
+                for (;;) {
+                    const uint32_t ex = rl_ld(sm.lab, x);
+                    const uint32_t q = ex & RL_IDX;
+                    if ((ex & RL_ROOT) || q == x) break;
+                    const uint32_t eq_ = rl_ld(sm.lab, q);
+                    const uint32_t g = eq_ & RL_IDX;
+                    if ((eq_ & RL_ROOT) || g == q) {
+                        x = q;
+                        break;
+                    }
+                    rl_st(sm.lab, x, g);
+                    x = g;
+                }
+                if (x != pc) rl_st(sm.lab, pc, x);
+                const int sh = rl_shift(x);
+                const uint32_t old = __hip_atomic_fetch_or(rl_word(sm.lab, x), RL_ROOT << sh, __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_WORKGROUP) >> sh;
+                if (!(old & RL_ROOT)) {
+                    const uint32_t f = ((old >> RL_FSHIFT) & 7u) | (anygs && rl_gs(owngs, x) ? (F_G | F_S) : 0u) | F_SEAM;
+                    uint32_t id = RL_NOID;
                     const uint32_t j = atomicAdd(&sm.count, 1u);
                     if (j >= (uint32_t)S) {
                         sm.err = 1u;
@@ -520,14 +578,17 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                         id = j;
                         ws.sbits[gtile * S + j] = (uint8_t)(f & 3u);
                     }
+                    rl_st(sm.lab, x, RL_ROOT | id << 1 | f << RL_FSHIFT);
                 }
-                rl_st(sm.lab, p, RL_ROOT | id << 1 | f << RL_FSHIFT);
             }
         }
         // ---- masks: each run takes its zone's evidence ----------------------
-        uint32_t zg = 0, zs = 0, zx = 0;   // at run starts: the zone's G / S / seam bits
+        // A GS run's zone has both kinds (masks 0, closed); every other run's
+        // entry is its root or names it (compressed above), and the root's
+        // evidence is its entry's bits plus both kinds if it is a GS run.
+        uint32_t zg = GSR, zs = GSR, zx = 0;   // at run starts: the zone's G / S / seam bits
         {
-            uint32_t m = RS;
+            uint32_t m = RS & ~GSR;
             while (m) {
                 const int b1 = __builtin_ctz(m);
                 m &= m - 1u;
@@ -535,13 +596,16 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                 m &= m - 1u;
                 const uint32_t p1 = nrow + (b1 << RL_COL), p2 = nrow + (b2 << RL_COL);
                 const uint32_t e1 = rl_ld(sm.lab, p1), e2 = rl_ld(sm.lab, p2);
-                // the roots' entries (read unconditionally: an index below 4096 either way)
-                const uint32_t r1 = rl_ld(sm.lab, e1 & RL_IDX), r2 = rl_ld(sm.lab, e2 & RL_IDX);
-                const uint32_t z1 = ((e1 & RL_ROOT) || (e1 & RL_IDX) == p1) ? e1 : r1;
-                const uint32_t z2 = ((e2 & RL_ROOT) || (e2 & RL_IDX) == p2) ? e2 : r2;
-                zg |= ((z1 >> RL_FSHIFT) & 1u) << b1 | ((z2 >> RL_FSHIFT) & 1u) << b2;
-                zs |= ((z1 >> (RL_FSHIFT + 1)) & 1u) << b1 | ((z2 >> (RL_FSHIFT + 1)) & 1u) << b2;
-                zx |= ((z1 >> (RL_FSHIFT + 2)) & 1u) << b1 | ((z2 >> (RL_FSHIFT + 2)) & 1u) << b2;
+                const uint32_t r1 = ((e1 & RL_ROOT) || (e1 & RL_IDX) == p1) ? p1 : (e1 & RL_IDX);
+                const uint32_t r2 = ((e2 & RL_ROOT) || (e2 & RL_IDX) == p2) ? p2 : (e2 & RL_IDX);
+                uint32_t z1 = rl_ld(sm.lab, r1) >> RL_FSHIFT, z2 = rl_ld(sm.lab, r2) >> RL_FSHIFT;
+                if (anygs) {
+                    z1 |= rl_gs(owngs, r1) * (F_G | F_S);
+                    z2 |= rl_gs(owngs, r2) * (F_G | F_S);
+                }
+                zg |= (z1 & 1u) << b1 | (z2 & 1u) << b2;
+                zs |= ((z1 >> 1) & 1u) << b1 | ((z2 >> 1) & 1u) << b2;
+                zx |= ((z1 >> 2) & 1u) << b1 | ((z2 >> 2) & 1u) << b2;
             }
         }
         const uint32_t mxs = RS & ~zg, mns = RS & ~zs;
@@ -564,31 +628,33 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
     __syncthreads();   // every root numbered
     if (!uniform) {
         // ---- perimeter records -------------------------------------------------
-        // A seam pixel's zone must have a fragment id (invariant code 2).
+        // Records are read only at seam pixels (the merges follow equal pixel
+        // pairs across tile edges), whose runs were compressed by their
+        // claims.  A seam pixel's zone must have a fragment id (invariant
+        // code 2).
         {
             // top row (lanes 0-31) and bottom row (lanes 32-63), column 32h + (y & 31)
             const int bl = y & 31, x = 32 * h + bl;
             const int ry = y < 32 ? 0 : th - 1;
             const uint32_t rs_r = __builtin_amdgcn_readlane(RS, 0), rs_b = __builtin_amdgcn_readlane(RS, th - 1);
-            const uint32_t ns_r = __builtin_amdgcn_readlane(NS, 0), ns_b = __builtin_amdgcn_readlane(NS, th - 1);
             const uint32_t sx_r = __builtin_amdgcn_readlane(SEAM, 0), sx_b = __builtin_amdgcn_readlane(SEAM, th - 1);
-            const uint32_t rsm = y < 32 ? rs_r : rs_b, nsm = y < 32 ? ns_r : ns_b, sxm = y < 32 ? sx_r : sx_b;
-            if (x < tw && bit(nsm, bl)) {
-                const uint32_t e = rl_zone(sm.lab, rnode(ry, 32 * h + high_at_or_below(rsm, bl)));
-                rec[(y < 32 ? 0 : RT) + x] = (uint8_t)rl_id(e);
-                if (bit(sxm, bl) && rl_id(e) == RL_NOID) sm.err = 2u;
+            const uint32_t rsm = y < 32 ? rs_r : rs_b, sxm = y < 32 ? sx_r : sx_b;
+            if (bit(sxm, bl)) {
+                const uint32_t id = rl_zone_id(rl_zone(sm.lab, rnode(ry, 32 * h + high_at_or_below(rsm, bl))));
+                rec[(y < 32 ? 0 : RT) + x] = (uint8_t)id;
+                if (id == RL_NOID) sm.err = 2u;
             }
             // left column (wave 0) and right column (the wave holding column tw - 1)
-            if (h == 0 && (NS & 1u)) {
-                const uint32_t e = rl_zone(sm.lab, nrow);
-                rec[2 * RT + y] = (uint8_t)rl_id(e);
-                if ((SEAM & 1u) && rl_id(e) == RL_NOID) sm.err = 2u;
+            if (h == 0 && (SEAM & 1u)) {
+                const uint32_t id = rl_zone_id(rl_zone(sm.lab, nrow));
+                rec[2 * RT + y] = (uint8_t)id;
+                if (id == RL_NOID) sm.err = 2u;
             }
             const int br = tw - 1 - 32 * h;
-            if (br >= 0 && br < 32 && bit(NS, br)) {
-                const uint32_t e = rl_zone(sm.lab, nrow + (high_at_or_below(RS, br) << RL_COL));
-                rec[3 * RT + y] = (uint8_t)rl_id(e);
-                if (bit(SEAM, br) && rl_id(e) == RL_NOID) sm.err = 2u;
+            if (br >= 0 && br < 32 && bit(SEAM, br)) {
+                const uint32_t id = rl_zone_id(rl_zone(sm.lab, nrow + (high_at_or_below(RS, br) << RL_COL)));
+                rec[3 * RT + y] = (uint8_t)id;
+                if (id == RL_NOID) sm.err = 2u;
             }
         }
         // ---- record plane (tiles with an open fragment) ----------------------
@@ -605,7 +671,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
             while (m) {
                 const int bb = __builtin_ctz(m);
                 m &= m - 1u;
-                const uint32_t id = rl_id(rl_zone(sm.lab, nrow + (bb << RL_COL))) & 0xFFu;
+                const uint32_t id = rl_zone_id(rl_zone(sm.lab, nrow + (bb << RL_COL))) & 0xFFu;
                 // the run: bb .. the first column without an equal right link
                 const int be = __builtin_ctz(~H & (0xFFFFFFFFu << bb));
                 int q = bb;

@@ -2,7 +2,8 @@
 // 64 x 64 tiles (values drawn from few levels so that zones of every shape
 // occur, random valid width and height) it builds the run union-find the
 // kernel builds -- runs of each 32-column half-row, the row_edges unions to
-// the half-row above, the three cross-half unions at columns 31 | 32 -- and
+// the half-row above, the cross-half unions at columns 31 | 32 that
+// cross_keep keeps -- and
 // compares its partition with a breadth-first labelling of the 8-connected
 // equal-value zones of the tile.  Also checks fill_fwd / fill_bwd against a
 // per-bit walk.  Prints the counts; exit status 1 on any mismatch.
@@ -63,6 +64,7 @@ static int check_fills() {
 int main() {
     const int fill_bad = check_fills();
     int tiles = 0, bad = 0;
+    long cross_dropped = 0;
     const int levels_set[] = {2, 3, 4, 8, 256};
     for (int it = 0; it < 3000; ++it) {
         const int levels = levels_set[it % 5];
@@ -136,13 +138,22 @@ int main() {
                         uf.unite(a, 64 * (y - 1) + 32 * h + high_at_or_below(RS[h][y - 1], b + 1));
                 }
             }
+        // cross-half relations, the redundant ones dropped (cross_keep)
+        auto eqv = [&](int y0, int x0, int y1, int x1) {
+            return valid(y0, x0) && valid(y1, x1) && px(y0, x0) == px(y1, x1) ? 1u : 0u;
+        };
+        int dropped = 0;
         for (int y = 0; y < 64; ++y) {
             if (!(XH[y] || XDR[y] || XUR[y])) continue;
+            const uint32_t keep = cross_keep(XH[y], XDR[y], XUR[y], y ? XH[y - 1] : 0u, eqv(y, 31, y + 1, 31),
+                                             eqv(y - 1, 31, y, 31));
+            dropped += (int)(XH[y] + XDR[y] + XUR[y]) - __builtin_popcount(keep);
             const int a = 64 * y + high_at_or_below(RS[0][y], 31);
-            if (XH[y]) uf.unite(a, 64 * y + 32);
-            if (XDR[y]) uf.unite(a, 64 * (y + 1) + 32);
-            if (XUR[y]) uf.unite(a, 64 * (y - 1) + 32);
+            if (keep & 1u) uf.unite(a, 64 * y + 32);
+            if (keep & 2u) uf.unite(a, 64 * (y + 1) + 32);
+            if (keep & 4u) uf.unite(a, 64 * (y - 1) + 32);
         }
+        cross_dropped += dropped;
         // breadth-first zones
         std::vector<int> zone(64 * 64, -1);
         std::vector<int> q;
@@ -193,6 +204,7 @@ int main() {
             if (bad < 5) std::printf("mismatch: tile %d levels %d tw %d th %d cell %d\n", it, levels, tw, th, cell);
         }
     }
-    std::printf("fills: %d mismatches; %d tiles, %d mismatches\n", fill_bad, tiles, bad);
+    std::printf("fills: %d mismatches; %d tiles, %d mismatches (%ld cross-half relations dropped)\n", fill_bad, tiles, bad,
+                cross_dropped);
     return (fill_bad || bad) ? 1 : 0;
 }
