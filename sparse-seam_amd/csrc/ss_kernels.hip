@@ -34,11 +34,11 @@ __global__ __launch_bounds__(NT) void k_tile_stage(Images ims, Ws ws) {
 template <bool MULTI>
 __global__ __launch_bounds__(RT_THREADS * RT_NTW) void k_row_tile(Images ims, Ws ws) {
     __shared__ RowTileSmem sm[RT_NTW];
-    const int sub = threadIdx.x / RT_THREADS;
+    const int sub = RT_NTW == 1 ? 0 : (int)(threadIdx.x / RT_THREADS);   // 0: the tile index stays scalar
     const int64_t gtile = xcd_block(blockIdx.x, gridDim.x) * RT_NTW + sub;
     const bool live = gtile < ws.ntiles;
     const ImgDev im = live ? locate<MULTI>(ims, ws.tile_img, gtile) : ims.one;
-    row_tile_stage<MULTI>(sm[sub], im, gtile, ws, live, threadIdx.x % RT_THREADS);
+    row_tile_stage<MULTI>(sm[sub], im, gtile, ws, live, RT_NTW == 1 ? (int)threadIdx.x : (int)(threadIdx.x % RT_THREADS));
 }
 
 // Per-tile, per-merge-block and per-seam-unit image index of a batch: one

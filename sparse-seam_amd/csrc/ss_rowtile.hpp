@@ -306,17 +306,53 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
     const int nb = min(max(tw - 32 * h, 0), 32);   // valid columns of this half
     const uint32_t VM = (y < th) ? (nb >= 32 ? 0xFFFFFFFFu : ((1u << nb) - 1u)) : 0u;
     const HalfRow a = read_half_row(sm, y - 1, h), b = read_half_row(sm, y, h), c = read_half_row(sm, y + 1, h);
+    // The 3 x 3 maximum and minimum are taken down the columns first (the
+    // three rows' 16-bit lanes of even pixels, bytes 0 / 2, and of odd
+    // pixels, bytes 1 / 3, of each dword), then across: pixel 0 of a dword
+    // needs column -1 (the previous dword's odd high lane), pixel 3 column 4
+    // (the next dword's even low lane).  Dwords 0 and 9 hold the columns left
+    // and right of the half: of these only the odd / even lanes are needed.
+    uint32_t pxo, pno;   // previous dword: odd max / min
+    {
+        const uint32_t sel = 0x0C030C01u;
+        const uint32_t ao = __builtin_amdgcn_perm(0u, a.w[0], sel), bo = __builtin_amdgcn_perm(0u, b.w[0], sel),
+                       co = __builtin_amdgcn_perm(0u, c.w[0], sel);
+        pxo = pk_max(pk_max(ao, bo), co);
+        pno = pk_min(pk_min(ao, bo), co);
+    }
+    auto vert = [&](int i, uint32_t& xe, uint32_t& xo, uint32_t& ne, uint32_t& no) {
+        const uint32_t ae = __builtin_amdgcn_perm(0u, a.w[i], 0x0C020C00u), ao = __builtin_amdgcn_perm(0u, a.w[i], 0x0C030C01u);
+        const uint32_t be = __builtin_amdgcn_perm(0u, b.w[i], 0x0C020C00u), bo = __builtin_amdgcn_perm(0u, b.w[i], 0x0C030C01u);
+        const uint32_t ce = __builtin_amdgcn_perm(0u, c.w[i], 0x0C020C00u), co = __builtin_amdgcn_perm(0u, c.w[i], 0x0C030C01u);
+        xe = pk_max(pk_max(ae, be), ce);
+        xo = pk_max(pk_max(ao, bo), co);
+        ne = pk_min(pk_min(ae, be), ce);
+        no = pk_min(pk_min(ao, bo), co);
+    };
+    uint32_t cxe, cxo, cne, cno;   // current dword: even / odd max / min
+    vert(1, cxe, cxo, cne, cno);
     uint32_t G = 0, Sm = 0, ER = 0, ED = 0, EDL = 0, EDR = 0;
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
-        const RowMinMax ma = row_minmax(a.w[d], a.w[d + 1], a.w[d + 2]);
-        const RowMinMax mb = row_minmax(b.w[d], b.w[d + 1], b.w[d + 2]);
-        const RowMinMax mc = row_minmax(c.w[d], c.w[d + 1], c.w[d + 2]);
-        // 3 x 3 maximum and minimum (the pixel itself included) back in byte
-        // form: a strictly greater neighbour exists iff the maximum differs
-        // from the pixel, a strictly smaller one iff the minimum does
-        const uint32_t mx3 = pk_max(pk_max(ma.maxe, mb.maxe), mc.maxe) | pk_max(pk_max(ma.maxo, mb.maxo), mc.maxo) << 8;
-        const uint32_t mn3 = pk_min(pk_min(ma.mine, mb.mine), mc.mine) | pk_min(pk_min(ma.mino, mb.mino), mc.mino) << 8;
+        uint32_t nxe, nxo = 0, nne, nno = 0;   // next dword
+        if (d < 7) {
+            vert(d + 2, nxe, nxo, nne, nno);
+        } else {
+            const uint32_t sel = 0x0C020C00u;
+            const uint32_t ae = __builtin_amdgcn_perm(0u, a.w[9], sel), be = __builtin_amdgcn_perm(0u, b.w[9], sel),
+                           ce = __builtin_amdgcn_perm(0u, c.w[9], sel);
+            nxe = pk_max(pk_max(ae, be), ce);
+            nne = pk_min(pk_min(ae, be), ce);
+        }
+        // lanes (pixel 0, pixel 2) take (column -1, column 1), lanes (pixel 1,
+        // pixel 3) take (column 2, column 4); back in byte form a strictly
+        // greater neighbour exists iff the maximum differs from the pixel, a
+        // strictly smaller one iff the minimum does
+        const uint32_t M = pk_max(cxe, cxo), N = pk_min(cne, cno);
+        const uint32_t mx3 = pk_max(M, __builtin_amdgcn_alignbit(cxo, pxo, 16)) |
+                             pk_max(M, __builtin_amdgcn_alignbit(nxe, cxe, 16)) << 8;
+        const uint32_t mn3 = pk_min(N, __builtin_amdgcn_alignbit(cno, pno, 16)) |
+                             pk_min(N, __builtin_amdgcn_alignbit(nne, cne, 16)) << 8;
         const uint32_t V = b.w[d + 1];
         const uint32_t eR = eq80(V, __builtin_amdgcn_alignbyte(b.w[d + 2], V, 1));
         const uint32_t eD = eq80(V, c.w[d + 1]);
@@ -328,6 +364,12 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         ED |= nib80(eD) << (4 * d);
         EDL |= nib80(eDL) << (4 * d);
         EDR |= nib80(eDR) << (4 * d);
+        pxo = cxo;
+        pno = cno;
+        cxe = nxe;
+        cxo = nxo;
+        cne = nne;
+        cno = nno;
     }
     G &= VM;
     Sm &= VM;
