@@ -709,13 +709,20 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                         spread4((outMN >> (4 * d)) & 0xFu) << 1;
             *reinterpret_cast<uint4*>(row) = make_uint4(cw[0], cw[1], cw[2], cw[3]);
             *reinterpret_cast<uint4*>(row + 16) = make_uint4(cw[4], cw[5], cw[6], cw[7]);
+            // the bytes of open runs after their first: filled from it
+            const uint32_t OF = fill_fwd(opens, H) & ~opens;
+            // A wave with long open runs writes each run's id at its first
+            // byte and fills the rest by doubling over the row's bytes (byte
+            // i takes byte i - s while both are fill bytes, s = 1, 2, 4, 8,
+            // 16); otherwise each run is written byte / dword-wise.
+            const bool dbl = __ballot(__builtin_popcount(OF) >= 12) != 0;
             uint32_t m = opens;
             while (m) {
                 const int bb = __builtin_ctz(m);
                 m &= m - 1u;
                 const uint32_t id = rl_zone_id(rl_zone(sm.lab, nrow + (bb << RL_COL))) & 0xFFu;
                 // the run: bb .. the first column without an equal right link
-                const int be = __builtin_ctz(~H & (0xFFFFFFFFu << bb));
+                const int be = dbl ? bb : __builtin_ctz(~H & (0xFFFFFFFFu << bb));
                 int q = bb;
                 while (q <= be) {
                     if ((q & 3) == 0 && q + 3 <= be) {
@@ -726,6 +733,32 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                         q += 1;
                     }
                 }
+            }
+            if (dbl) {
+                const uint4 r0 = *reinterpret_cast<const uint4*>(row), r1 = *reinterpret_cast<const uint4*>(row + 16);
+                uint32_t v[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+                uint32_t f[8];
+#pragma unroll
+                for (int d = 0; d < 8; ++d) f[d] = mask_bytes(OF, d);
+#pragma unroll
+                for (int sb = 1; sb <= 16; sb *= 2) {
+#pragma unroll
+                    for (int d = 7; d >= 0; --d) {
+                        uint32_t vs, fs;   // bytes 4d + j - sb
+                        if (sb < 4) {
+                            const uint32_t vp = d > 0 ? v[d - 1] : 0u, fp = d > 0 ? f[d - 1] : 0u;
+                            vs = __builtin_amdgcn_alignbyte(v[d], vp, 4 - sb);
+                            fs = __builtin_amdgcn_alignbyte(f[d], fp, 4 - sb);
+                        } else {
+                            vs = d >= sb / 4 ? v[d - sb / 4] : 0u;
+                            fs = d >= sb / 4 ? f[d - sb / 4] : 0u;
+                        }
+                        v[d] = (f[d] & vs) | (~f[d] & v[d]);
+                        f[d] &= fs;
+                    }
+                }
+                *reinterpret_cast<uint4*>(row) = make_uint4(v[0], v[1], v[2], v[3]);
+                *reinterpret_cast<uint4*>(row + 16) = make_uint4(v[4], v[5], v[6], v[7]);
             }
         }
     } else if (live) {
