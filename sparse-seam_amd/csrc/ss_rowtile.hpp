@@ -71,9 +71,13 @@ struct RowTileSmem {
     uint32_t count, open, err, zflags, nonuni, anygs;
 };
 
+// Entry loads are plain (an atomic 16-bit load costs a masking instruction
+// after every ds_read_u16): every walk reads a different entry each step, and
+// a stale read is harmless where the entries may change meanwhile (a find
+// that stops at a node that is no longer a root has its CAS fail and walks
+// again; an evidence test that misses bits only repeats an atomic OR).
 __device__ __forceinline__ uint32_t rl_ld(const uint16_t* lab, uint32_t o) {
-    return __hip_atomic_load(reinterpret_cast<const uint16_t*>(reinterpret_cast<const uint8_t*>(lab) + o),
-                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return *reinterpret_cast<const uint16_t*>(reinterpret_cast<const uint8_t*>(lab) + o);
 }
 __device__ __forceinline__ void rl_st(uint16_t* lab, uint32_t o, uint32_t v) {
     __hip_atomic_store(reinterpret_cast<uint16_t*>(reinterpret_cast<uint8_t*>(lab) + o), (uint16_t)v,
