@@ -530,13 +530,15 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         }
     }
     __syncthreads();
+    uint32_t gsr = 0;   // runs (not GS runs) whose root is a GS run: zone with both kinds
     if (!uniform) {
         // ---- evidence of the other runs at their roots; compression ---------
         // A GS run's zone is rooted at a GS run, which tells its evidence, so
-        // only the runs that are not GS runs report theirs.  Two runs per
-        // iteration (the second repeats the first when the row has none
-        // left), their two finds walked together: the dependent LDS loads of
-        // one overlap those of the other.
+        // only the runs that are not GS runs report theirs, and one whose
+        // root turns out to be a GS run need not (its masks are 0, its zone
+        // closed).  Two runs per iteration (the second repeats the first when
+        // the row has none left), their two finds walked together: the
+        // dependent LDS loads of one overlap those of the other.
         uint32_t m = RS & ~GSR;
         while (m) {
             const int b1 = __builtin_ctz(m);
@@ -564,11 +566,17 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
             if (x2 != p2) rl_st(sm.lab, p2, x2);
             const uint32_t f1 = bit(rfg, b1) | bit(rfs, b1) << 1 | bit(rfx, b1) << 2;
             const uint32_t f2 = bit(rfg, b2) | bit(rfs, b2) << 1 | bit(rfx, b2) << 2;
+            uint32_t g1 = 0, g2 = 0;
+            if (anygs) {
+                g1 = rl_gs(owngs, x1);
+                g2 = rl_gs(owngs, x2);
+                gsr |= g1 << b1 | g2 << b2;
+            }
             const uint32_t h1 = rl_ld(sm.lab, x1) >> RL_FSHIFT, h2 = rl_ld(sm.lab, x2) >> RL_FSHIFT;
-            if (f1 & ~h1)
+            if (!g1 && (f1 & ~h1))
                 __hip_atomic_fetch_or(rl_word(sm.lab, x1), f1 << (RL_FSHIFT + rl_shift(x1)), __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (f2 & ~h2)
+            if (!g2 && (f2 & ~h2))
                 __hip_atomic_fetch_or(rl_word(sm.lab, x2), f2 << (RL_FSHIFT + rl_shift(x2)), __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_WORKGROUP);
         }
@@ -629,12 +637,13 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
             }
         }
         // ---- masks: each run takes its zone's evidence ----------------------
-        // A GS run's zone has both kinds (masks 0, closed); every other run's
-        // entry is its root or names it (compressed above), and the root's
-        // evidence is its entry's bits plus both kinds if it is a GS run.
-        uint32_t zg = GSR, zs = GSR, zx = 0;   // at run starts: the zone's G / S / seam bits
+        // A zone rooted at a GS run has both kinds (masks 0, closed): the GS
+        // runs and gsr.  Every other run's entry is its root or names it
+        // (compressed above), a root that is not a GS run, whose entry holds
+        // the zone's evidence.
+        uint32_t zg = GSR | gsr, zs = GSR | gsr, zx = 0;   // at run starts: the zone's G / S / seam bits
         {
-            uint32_t m = RS & ~GSR;
+            uint32_t m = RS & ~(GSR | gsr);
             while (m) {
                 const int b1 = __builtin_ctz(m);
                 m &= m - 1u;
@@ -644,11 +653,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                 const uint32_t e1 = rl_ld(sm.lab, p1), e2 = rl_ld(sm.lab, p2);
                 const uint32_t r1 = ((e1 & RL_ROOT) || (e1 & RL_IDX) == p1) ? p1 : (e1 & RL_IDX);
                 const uint32_t r2 = ((e2 & RL_ROOT) || (e2 & RL_IDX) == p2) ? p2 : (e2 & RL_IDX);
-                uint32_t z1 = rl_ld(sm.lab, r1) >> RL_FSHIFT, z2 = rl_ld(sm.lab, r2) >> RL_FSHIFT;
-                if (anygs) {
-                    z1 |= rl_gs(owngs, r1) * (F_G | F_S);
-                    z2 |= rl_gs(owngs, r2) * (F_G | F_S);
-                }
+                const uint32_t z1 = rl_ld(sm.lab, r1) >> RL_FSHIFT, z2 = rl_ld(sm.lab, r2) >> RL_FSHIFT;
                 zg |= (z1 & 1u) << b1 | (z2 & 1u) << b2;
                 zs |= ((z1 >> 1) & 1u) << b1 | ((z2 >> 1) & 1u) << b2;
                 zx |= ((z1 >> 2) & 1u) << b1 | ((z2 >> 2) & 1u) << b2;
