@@ -99,6 +99,7 @@ static_assert(sizeof(WsHeader) == 256, "header must be 256 bytes");
 constexpr uint32_t COUNT_MASK = 0x00FFFFFFu;
 constexpr uint32_t CNT_UNIFORM = 1u << 24;   // the tile is one zone (fragment 0 if seam)
 constexpr uint32_t CNT_OPEN = 1u << 25;      // the tile wrote its per-pixel record plane
+constexpr uint32_t CNT_ROWS = 1u << 26;      // only the plane rows in orows[t] were written
 constexpr int COUNT_ERR_SHIFT = 28;
 // Record-plane byte of a pixel outside every open fragment: REC_CLOSED plus
 // its final mask bits (bit 0 max, bit 1 min).  Fragment ids stay below it
@@ -114,6 +115,8 @@ struct Ws {
     uint8_t* sbits;      // [tiles * S] local G/S evidence per fragment slot
     uint8_t* rec8;       // [tiles * T*T] per-pixel record (open tiles only): the
                          //   pixel's open fragment, or REC_CLOSED | its final masks
+    uint64_t* orows;     // [tiles] with CNT_ROWS: the record-plane rows that hold an
+                         //   open pixel (bit r = tile row r), the only rows written
     uint8_t* rec;        // [tiles * 4T] fragment id of seam perimeter pixels
                          //   [0,T) top row, [T,2T) bottom row, [2T,3T) left col, [3T,4T) right col
     uint8_t* eq;         // [tiles * 2T] cross-seam equality bits: [0,T) right col, [T,2T) bottom row

@@ -20,7 +20,7 @@ struct Layout {
     int slots = 0;
     uint32_t nblocks = 0;
     size_t off_counts = 0, off_base = 0, off_bsum = 0, off_berr = 0, off_sbits = 0,
-           off_rec8 = 0, off_rec = 0, off_eq = 0, off_dparent = 0, off_dbits = 0,
+           off_rec8 = 0, off_orows = 0, off_rec = 0, off_eq = 0, off_dparent = 0, off_dbits = 0,
            off_descs = 0, off_tile_img = 0, off_block_img = 0, off_seam_img = 0;
     size_t bytes = 0;
 };
@@ -48,6 +48,7 @@ inline Layout make_layout(int64_t ntiles, int64_t nblk, int64_t nseam, int T, in
     L.off_berr = take((size_t)L.nblocks * 4 + 4);
     L.off_sbits = take(nt * S);
     L.off_rec8 = take(nt * (size_t)T * (size_t)T);
+    L.off_orows = take(nt * 8);
     L.off_rec = take(nt * 4 * (size_t)T);
     L.off_eq = take(nt * 2 * (size_t)T);
     L.off_dparent = take(nt * S * 4);
@@ -70,6 +71,7 @@ inline Ws make_ws(void* base, const Layout& L, int threads) {
     w.berr = reinterpret_cast<uint32_t*>(b + L.off_berr);
     w.sbits = reinterpret_cast<uint8_t*>(b + L.off_sbits);
     w.rec8 = reinterpret_cast<uint8_t*>(b + L.off_rec8);
+    w.orows = reinterpret_cast<uint64_t*>(b + L.off_orows);
     w.rec = reinterpret_cast<uint8_t*>(b + L.off_rec);
     w.eq = reinterpret_cast<uint8_t*>(b + L.off_eq);
     w.dparent = reinterpret_cast<uint32_t*>(b + L.off_dparent);

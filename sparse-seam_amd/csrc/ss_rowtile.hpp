@@ -68,7 +68,7 @@ struct RowTileSmem {
         alignas(16) uint8_t plane[RT * RT_P];           // record plane (row pitch RT_P), built after the staged rows are read
     };
     alignas(16) uint16_t lab[RT * RT];
-    uint32_t count, open, err, zflags, nonuni, anygs;
+    uint32_t count, open, err, zflags, nonuni, anygs, orow[2];
 };
 
 // Entry loads are plain (an atomic 16-bit load costs a masking instruction
@@ -301,6 +301,8 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         sm.err = 0;
         sm.zflags = 0;
         sm.nonuni = 0;
+        sm.orow[0] = 0;
+        sm.orow[1] = 0;
         sm.anygs = 0;
     }
     if (live) row_tile_load(sm, im, x0, y0, tw, th, tid);
@@ -663,6 +665,14 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         // a fragment that already has both kinds of evidence is final
         opens = zx & ~(zg & zs);
         if (opens) sm.open = 1u;
+        {
+            // the tile rows holding an open pixel (either half)
+            const uint64_t orw = __ballot(opens != 0u);
+            if (y == 0 && orw) {
+                atomicOr(&sm.orow[0], (uint32_t)orw);
+                atomicOr(&sm.orow[1], (uint32_t)(orw >> 32));
+            }
+        }
         const uint32_t MX = ((SGL & ~G) | fill_fwd(mxs, H)) & VM;
         const uint32_t MN = ((SGL & ~Sm) | fill_fwd(mns, H)) & VM;
         outMX = MX;
@@ -786,14 +796,19 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
     reinterpret_cast<uint2*>(sm.lab)[tid] = make_uint2(outMX, outMN);
     const uint32_t count = sm.count;
     const bool open = !uniform && sm.open != 0;
+    const uint64_t orows = (uint64_t)sm.orow[1] << 32 | sm.orow[0];
     if (open) {
+        // only the rows with an open pixel: the others hold closed pixels,
+        // which the write-back never changes
         uint8_t* plane = ws.rec8 + gtile * (int64_t)(RT * RT);
         uint4* dst = reinterpret_cast<uint4*>(plane);
 #pragma unroll
         for (int i = 0; i < RT * RT / 16 / RT_THREADS; ++i) {
             const int c = tid + i * RT_THREADS;   // 16-byte chunk: row c / 4, part c % 4
-            dst[c] = *reinterpret_cast<const uint4*>(sm.plane + (c >> 2) * RT_P + 16 * (c & 3));
+            if ((orows >> (c >> 2)) & 1u)
+                dst[c] = *reinterpret_cast<const uint4*>(sm.plane + (c >> 2) * RT_P + 16 * (c & 3));
         }
+        if (tid == 0) ws.orows[gtile] = orows;
     }
     if (count) {
         uint8_t* eq = ws.eq + gtile * 2 * RT;
@@ -801,7 +816,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         if (y < 32) eq[RT + 32 * h + y] = (uint8_t)eqb;
     }
     if (tid == 0 && live)
-        ws.counts[gtile] = min(count, (uint32_t)S) | (uniform ? CNT_UNIFORM : 0u) | (open ? CNT_OPEN : 0u) |
+        ws.counts[gtile] = min(count, (uint32_t)S) | (uniform ? CNT_UNIFORM : 0u) | (open ? CNT_OPEN | CNT_ROWS : 0u) |
                            (sm.err << COUNT_ERR_SHIFT);
     __syncthreads();
     // 16-pixel chunk c: tile row c / 4, columns 16 (c % 4) .. + 15

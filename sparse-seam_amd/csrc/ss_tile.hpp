@@ -996,9 +996,13 @@ __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, i
     const uint8_t* plane = ws.rec8 + gtile * (int64_t)(T * T);
     constexpr int VR = T / 16;   // 16-byte vectors per row
     constexpr uint32_t CL4 = REC_CLOSED * 0x01010101u;
+    // a tile stage that wrote only the rows with an open pixel says which
+    // (the others hold closed pixels, which cannot change)
+    const uint64_t rows = (cw & CNT_ROWS) ? ws.orows[gtile] : ~0ull;
     for (int u = lane; u < T * T / 16; u += 64) {
         const int ly = u / VR, c0 = 16 * (u % VR);
-        const uint4 rv = *reinterpret_cast<const uint4*>(plane + ly * T + c0);
+        const uint4 rv = ((rows >> ly) & 1u) ? *reinterpret_cast<const uint4*>(plane + ly * T + c0)
+                                             : make_uint4(CL4, CL4, CL4, CL4);
         uint32_t rw[4] = {rv.x, rv.y, rv.z, rv.w};
         uint32_t mxw[4], mnw[4];
         uint32_t ch = 0;   // bit 6: a max value changed, bit 7: a min value
