@@ -41,6 +41,10 @@ constexpr int RT_THREADS = 128;   // two waves per tile
 constexpr int RT_NTW = 1;         // tiles per workgroup
 constexpr int RT_P = 80;          // staged row pitch: tile column c of staged row r at r * RT_P + 16 + c
 constexpr int RT_ROWS = 66;       // staged rows: tile rows -1 .. 64
+// Union queues (two waves) in the staged rows' LDS after the GS words
+// (the rows are no longer read when the unions run).
+constexpr int RT_UQ_OFF = 512;
+constexpr int RT_UQ = 592;        // node pairs per wave
 
 // Union-find nodes are indexed column-major, index 64 x + y for pixel (y, x),
 // so the entries the 64 lanes (rows) touch at one column are consecutive
@@ -67,8 +71,9 @@ struct RowTileSmem {
         alignas(16) uint8_t img[RT_ROWS * RT_P + 16];   // staged rows (+ the last row's right halo dword)
         alignas(16) uint8_t plane[RT * RT_P];           // record plane (row pitch RT_P), built after the staged rows are read
     };
+static_assert(RT_UQ_OFF + 2 * RT_UQ * 4 <= RT_ROWS * RT_P + 16, "union queues must fit in the staged rows");
     alignas(16) uint16_t lab[RT * RT];
-    uint32_t count, open, err, zflags, nonuni, anygs, orow[2];
+    uint32_t count, open, err, zflags, nonuni, anygs, orow[2], qn[2];
 };
 
 // Entry loads are plain (an atomic 16-bit load costs a masking instruction
@@ -303,6 +308,8 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         sm.nonuni = 0;
         sm.orow[0] = 0;
         sm.orow[1] = 0;
+        sm.qn[0] = 0;
+        sm.qn[1] = 0;
         sm.anygs = 0;
     }
     if (live) row_tile_load(sm, im, x0, y0, tw, th, tid);
@@ -513,22 +520,58 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
     __syncthreads();
     const bool anygs = sm.anygs != 0u;
     if (!uniform) {
-        // One loop over every union of the lane: the first edges without a
-        // plain link, the second edges, then (wave 0) the cross-half
-        // relations that cross_keep keeps, at the run through column 31.
+        // A lane's unions: the first edges without a plain link, the second
+        // edges, then (wave 0) the cross-half relations that cross_keep
+        // keeps, at the run through column 31.  Their number varies a lot
+        // across the wave and a union costs over a hundred instructions, so
+        // each lane appends its node pairs to a per-wave LDS queue and the
+        // lanes then take the queue 64 at a time (a lane whose pairs do not
+        // fit unites them itself).  A wave whose lanes have at most two each
+        // skips the queue.
         uint32_t tf = e.first & ~link1, ts = e.second, tx = xkeep;
-        while (tf | ts | tx) {
+        auto next_pair = [&](uint32_t& na, uint32_t& nb) {
             const bool isf = tf != 0u, iss = !isf && ts != 0u, isx = !isf && !iss;
             const uint32_t cur = isf ? tf : ts;
             const int bb = isx ? 31 : __builtin_ctz(cur);
             const int k = isf ? bb - (int)bit(e.kl, bb) + (int)bit(e.kr, bb) : bb + 1;
             const int j = __builtin_ctz(tx | 8u);   // 0: XH, 1: XDR, 2: XUR
-            const uint32_t nb = isx ? rnode(y, 32) + (j == 1 ? RL_ROW : 0u) - (j == 2 ? RL_ROW : 0u)
-                                    : nup + (high_at_or_below(RSup, k) << RL_COL);
+            nb = isx ? rnode(y, 32) + (j == 1 ? RL_ROW : 0u) - (j == 2 ? RL_ROW : 0u)
+                     : nup + (high_at_or_below(RSup, k) << RL_COL);
+            na = nrow + (high_at_or_below(RS, bb) << RL_COL);
             tf = isf ? tf & (tf - 1u) : tf;
             ts = iss ? ts & (ts - 1u) : ts;
             tx = isx ? tx & (tx - 1u) : tx;
-            rl_unite(sm.lab, owngs, anygs, nrow + (high_at_or_below(RS, bb) << RL_COL), nb);
+        };
+        const uint32_t n = __builtin_popcount(tf) + __builtin_popcount(ts) + __builtin_popcount(tx);
+        if (__ballot(n >= 3u)) {
+            uint32_t* uq = reinterpret_cast<uint32_t*>(sm.img + RT_UQ_OFF) + h * RT_UQ;
+            uint32_t slot = atomicAdd(&sm.qn[h], n);
+            while (tf | ts | tx) {
+                uint32_t na, nb;
+                next_pair(na, nb);
+                if (slot < (uint32_t)RT_UQ)
+                    uq[slot] = na | nb << 16;
+                else
+                    rl_unite(sm.lab, owngs, anygs, na, nb);
+                ++slot;
+            }
+            // the wave's appends are done (LDS operations of one wave complete
+            // in order; the fences keep the compiler from moving the reads up)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const uint32_t tot = min(sm.qn[h], (uint32_t)RT_UQ);
+            for (uint32_t i = (uint32_t)y; i < tot; i += 64) {
+                const uint32_t w = uq[i];
+                rl_unite(sm.lab, owngs, anygs, w & 0xFFFFu, w >> 16);
+            }
+        } else {
+            // at most two per lane: each lane unites its own
+            while (tf | ts | tx) {
+                uint32_t na, nb;
+                next_pair(na, nb);
+                rl_unite(sm.lab, owngs, anygs, na, nb);
+            }
         }
     }
     __syncthreads();
