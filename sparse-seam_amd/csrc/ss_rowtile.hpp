@@ -837,6 +837,14 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
     // that each store covers whole 64-byte tile rows: a lane's own row would
     // make every store instruction touch 64 rows.
     reinterpret_cast<uint2*>(sm.lab)[tid] = make_uint2(outMX, outMN);
+    // byte -> eight 0x00 / 0xFF bytes, for the mask stores (entries 2 tid, 2 tid + 1)
+    uint2* const bytes8 = reinterpret_cast<uint2*>(sm.lab) + RT_THREADS;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const uint32_t v = 2 * tid + k;
+        bytes8[v] = make_uint2(((((v & 0xFu) * 0x00204081u) & 0x01010101u) * 0xFFu),
+                               ((((v >> 4) * 0x00204081u) & 0x01010101u) * 0xFFu));
+    }
     const uint32_t count = sm.count;
     const bool open = !uniform && sm.open != 0;
     const uint64_t orows = (uint64_t)sm.orow[1] << 32 | sm.orow[0];
@@ -871,8 +879,10 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         const uint32_t mx16 = bm.x >> (16 * (q & 1)), mn16 = bm.y >> (16 * (q & 1));
         const int64_t o = (int64_t)(y0 + r) * im.opitch + x0 + 16 * q;
         if (16 * q + 16 <= tw) {
-            store_u128(im.mx + o, make_uint4(mask_bytes(mx16, 0), mask_bytes(mx16, 1), mask_bytes(mx16, 2), mask_bytes(mx16, 3)));
-            store_u128(im.mn + o, make_uint4(mask_bytes(mn16, 0), mask_bytes(mn16, 1), mask_bytes(mn16, 2), mask_bytes(mn16, 3)));
+            const uint2 x0b = bytes8[mx16 & 0xFFu], x1b = bytes8[(mx16 >> 8) & 0xFFu];
+            const uint2 n0b = bytes8[mn16 & 0xFFu], n1b = bytes8[(mn16 >> 8) & 0xFFu];
+            store_u128(im.mx + o, make_uint4(x0b.x, x0b.y, x1b.x, x1b.y));
+            store_u128(im.mn + o, make_uint4(n0b.x, n0b.y, n1b.x, n1b.y));
         } else {
             for (int k = 0; 16 * q + k < tw; ++k) {
                 im.mx[o + k] = (uint8_t)(bit(mx16, k) * 0xFFu);
