@@ -169,6 +169,10 @@ __device__ __forceinline__ uint32_t from_lane_above(uint32_t v) {
 
 // Top bits of a 0x80-per-byte flag word -> 4-bit mask (bit k = byte k).
 __device__ __forceinline__ uint32_t nib80(uint32_t f) { return (f * 0x00204081u) >> 28; }
+// Flags at bits 3 + 8k (bits 0-3 out) and 7 + 8k (bits 4-7 out) -> 8-bit mask.
+// The multiplier's four partial products of the eight flags fall on distinct
+// bits (no carries), bits 24-31 in order.
+__device__ __forceinline__ uint32_t nib80x2(uint32_t f) { return (f * 0x00204081u) >> 24; }
 // Bits 4d .. 4d + 3 of m -> 0x00 / 0xFF bytes.
 __device__ __forceinline__ uint32_t mask_bytes(uint32_t m, int d) {
     return __builtin_amdgcn_perm(0u, 0u, 0x0C0C0C0Cu | spread4((m >> (4 * d)) & 0xFu));
@@ -345,6 +349,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
     uint32_t cxe, cxo, cne, cno;   // current dword: even / odd max / min
     vert(1, cxe, cxo, cne, cno);
     uint32_t G = 0, Sm = 0, ER = 0, ED = 0, EDL = 0, EDR = 0;
+    uint32_t pG = 0, pS = 0, pR = 0, pD = 0, pDL = 0, pDR = 0;   // flags of the even dword of a pair
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
         uint32_t nxe, nxo = 0, nne, nno = 0;   // next dword
@@ -371,12 +376,24 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         const uint32_t eD = eq80(V, c.w[d + 1]);
         const uint32_t eDL = eq80(V, __builtin_amdgcn_alignbyte(c.w[d + 1], c.w[d], 3));
         const uint32_t eDR = eq80(V, __builtin_amdgcn_alignbyte(c.w[d + 2], c.w[d + 1], 1));
-        G |= nib80(ne_top(mx3, V)) << (4 * d);
-        Sm |= nib80(ne_top(mn3, V)) << (4 * d);
-        ER |= nib80(eR) << (4 * d);
-        ED |= nib80(eD) << (4 * d);
-        EDL |= nib80(eDL) << (4 * d);
-        EDR |= nib80(eDR) << (4 * d);
+        const uint32_t fG = ne_top(mx3, V), fS = ne_top(mn3, V);
+        if (d & 1) {
+            // two dwords' flags (the earlier one's moved to bit 3 of each
+            // byte) in one multiply: eight mask bits
+            G |= nib80x2((pG >> 4) | fG) << (8 * (d >> 1));
+            Sm |= nib80x2((pS >> 4) | fS) << (8 * (d >> 1));
+            ER |= nib80x2((pR >> 4) | eR) << (8 * (d >> 1));
+            ED |= nib80x2((pD >> 4) | eD) << (8 * (d >> 1));
+            EDL |= nib80x2((pDL >> 4) | eDL) << (8 * (d >> 1));
+            EDR |= nib80x2((pDR >> 4) | eDR) << (8 * (d >> 1));
+        } else {
+            pG = fG;
+            pS = fS;
+            pR = eR;
+            pD = eD;
+            pDL = eDL;
+            pDR = eDR;
+        }
         pxo = cxo;
         pno = cno;
         cxe = nxe;
@@ -584,6 +601,11 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         // closed).  Two runs per iteration (the second repeats the first when
         // the row has none left), their two finds walked together: the
         // dependent LDS loads of one overlap those of the other.
+        // the index mask in a register the compiler cannot see through: with
+        // the constant it narrows the walk to 16-bit compares and then widens
+        // every stored parent again
+        uint32_t idx = RL_IDX;
+        asm volatile("" : "+v"(idx));
         uint32_t m = RS & ~GSR;
         while (m) {
             const int b1 = __builtin_ctz(m);
@@ -595,8 +617,8 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
             // is the parent itself
             uint32_t x1 = p1, x2 = p2;
             for (;;) {
-                const uint32_t q1 = rl_ld(sm.lab, x1) & RL_IDX, q2 = rl_ld(sm.lab, x2) & RL_IDX;
-                const uint32_t g1 = rl_ld(sm.lab, q1) & RL_IDX, g2 = rl_ld(sm.lab, q2) & RL_IDX;
+                const uint32_t q1 = rl_ld(sm.lab, x1) & idx, q2 = rl_ld(sm.lab, x2) & idx;
+                const uint32_t g1 = rl_ld(sm.lab, q1) & idx, g2 = rl_ld(sm.lab, q2) & idx;
                 const bool d1 = g1 == q1, d2 = g2 == q2;
                 if (d1 && d2) {
                     x1 = q1;
@@ -618,10 +640,10 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                 gsr |= g1 << b1 | g2 << b2;
             }
             const uint32_t h1 = rl_ld(sm.lab, x1) >> RL_FSHIFT, h2 = rl_ld(sm.lab, x2) >> RL_FSHIFT;
-            if (!g1 && (f1 & ~h1))
+            if (!g1 && (f1 & (h1 ^ 7u)))   // h1 is at most 7: the bits its root lacks
                 __hip_atomic_fetch_or(rl_word(sm.lab, x1), f1 << (RL_FSHIFT + rl_shift(x1)), __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (!g2 && (f2 & ~h2))
+            if (!g2 && (f2 & (h2 ^ 7u)))
                 __hip_atomic_fetch_or(rl_word(sm.lab, x2), f2 << (RL_FSHIFT + rl_shift(x2)), __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_WORKGROUP);
         }
