@@ -110,8 +110,7 @@ __device__ __forceinline__ uint32_t rl_find_h(uint16_t* lab, uint32_t x) {
 // Run starts whose run holds both a strictly greater and a strictly smaller
 // neighbour ("GS runs") are marked per row and half in a 32-bit word of an
 // LDS array (owngs[2 y + h], bit b = column 32 h + b).  A node's key orders
-// GS runs before the others, then by row, then by column:
-//   key(o) = (GS ? 0 : 1) << 12 | y << 6 | x.
+// GS runs before the others, then by row, then by column.
 // Every link goes from a larger key to a smaller one, so the root of a tree
 // holds the smallest key in it: a zone's root is a GS run whenever the zone
 // has one, and such a zone has both kinds of evidence.
@@ -119,10 +118,14 @@ __device__ __forceinline__ uint32_t rl_gs(const uint32_t* owngs, uint32_t o) {
     const uint32_t y = (o >> 1) & 63u, x = o >> 7;
     return (owngs[2 * y + (x >> 5)] >> (x & 31u)) & 1u;
 }
-__device__ __forceinline__ uint32_t rl_key(const uint32_t* owngs, bool anygs, uint32_t o) {
-    // without GS runs in the tile every key has the same top bit
-    const uint32_t rm = (((o >> 1) & 63u) << 6) | (o >> 7);
-    return anygs ? ((rl_gs(owngs, o) ^ 1u) << 12) | rm : rm;
+// Row before column: the offset rotated right by 7 bits is y << 26 | x.
+__device__ __forceinline__ uint32_t rl_rowmajor(uint32_t o) { return __builtin_amdgcn_alignbit(o, o, 7); }
+// Whether a's key is larger than b's.
+__device__ __forceinline__ bool rl_key_gt(const uint32_t* owngs, bool anygs, uint32_t a, uint32_t b) {
+    const uint32_t ra = rl_rowmajor(a), rb = rl_rowmajor(b);
+    if (!anygs) return ra > rb;   // without GS runs in the tile every key has the same top bit
+    const uint32_t ga = rl_gs(owngs, a), gb = rl_gs(owngs, b);
+    return ga != gb ? ga < gb : ra > rb;
 }
 
 // Union of two trees: the root with the larger key is linked under the
@@ -135,7 +138,7 @@ __device__ __forceinline__ void rl_unite(uint16_t* lab, const uint32_t* owngs, b
         a = rl_find_h(lab, a);
         b = rl_find_h(lab, b);
         if (a == b) return;
-        const bool ahi = rl_key(owngs, anygs, a) > rl_key(owngs, anygs, b);
+        const bool ahi = rl_key_gt(owngs, anygs, a, b);
         const uint32_t hi = ahi ? a : b, lo = ahi ? b : a;
         uint32_t* w = rl_word(lab, hi);
         const int sh = rl_shift(hi);
