@@ -20,13 +20,16 @@
 //                   pixel of a tile edge on a block boundary; the edges of a
 //                   block seam are consecutive and a workgroup unites each
 //                   distinct pair of block roots once.
-//   4. resolve      full compression and OR of the local evidence into roots.
+//   4. resolve      full compression and OR of the local evidence into roots;
+//                   then one thread per fragment turns its zone's resolved
+//                   evidence into the fragment's final masks and change bits
+//                   (one byte per dense id).
 //   5. mark and     one wave per tile holding a fragment that may still change:
-//      write-back   fragments whose resolved evidence differs from the local
-//                   evidence are found; if any, the tile streams its 1-byte
-//                   per-pixel record plane (written in step 1 only by such
-//                   tiles: fragment ids, and the final masks of the other
-//                   pixels) and rewrites every tile row holding a changed pixel.
+//      write-back   the tile's change bytes are read; if any fragment changed,
+//                   the tile streams its 1-byte per-pixel record plane
+//                   (written in step 1 only by such tiles: fragment ids, and
+//                   the final masks of the other pixels) and rewrites every
+//                   tile row holding a changed pixel.
 //
 // Everything is exact integer work; no MFMA (nothing here is a contraction).
 #pragma once
@@ -112,7 +115,11 @@ struct Ws {
     uint32_t* base;      // [tiles] dense id of the tile's first fragment
     uint32_t* bsum;      // [nblocks + 1] scan block sums (last = total)
     uint32_t* berr;      // [nblocks] OR of the tile invariant codes per scan block
-    uint8_t* sbits;      // [tiles * S] local G/S evidence per fragment slot
+    uint8_t* sbits;      // [tiles * S] local G/S evidence per fragment slot (read by the
+                         //   block merge only)
+    uint8_t* flut;       // [tiles * S] per dense id, once resolved: final max / min mask
+                         //   (bits 0 / 1) and whether each changed (bits 6 / 7); the
+                         //   sbits memory, dead by then
     uint8_t* rec8;       // [tiles * T*T] per-pixel record (open tiles only): the
                          //   pixel's open fragment, or REC_CLOSED | its final masks
     uint64_t* orows;     // [tiles] with CNT_ROWS: the record-plane rows that hold an

@@ -70,6 +70,7 @@ inline Ws make_ws(void* base, const Layout& L, int threads) {
     w.bsum = reinterpret_cast<uint32_t*>(b + L.off_bsum);
     w.berr = reinterpret_cast<uint32_t*>(b + L.off_berr);
     w.sbits = reinterpret_cast<uint8_t*>(b + L.off_sbits);
+    w.flut = w.sbits;
     w.rec8 = reinterpret_cast<uint8_t*>(b + L.off_rec8);
     w.orows = reinterpret_cast<uint64_t*>(b + L.off_orows);
     w.rec = reinterpret_cast<uint8_t*>(b + L.off_rec);

@@ -554,6 +554,26 @@ __global__ __launch_bounds__(256) void k_resolve(Ws ws) {
     }
 }
 
+// ---- final per-fragment results -------------------------------------------
+
+// One thread per fragment (dense id), once every root holds its component's
+// evidence: the fragment's final masks and which of them differ from what the
+// tile stage wrote (it wrote the masks of the fragment's own evidence, so a
+// mask changes where the zone has evidence the fragment lacks).  Computing
+// these densely keeps the union-find walks off the write-back's per-tile
+// critical path: there a tile's bytes are one load.
+__global__ __launch_bounds__(256) void k_final(Ws ws) {
+    const uint32_t total = ws.hdr->total;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const uint32_t own = ws.dbits[i];
+        uint32_t res = own & 3u;
+        if (res != 3u) res |= (ws.dbits[gfind_ro(ws.dparent, (uint32_t)i)] >> 2) & 3u;
+        const uint32_t gain = res & ~own & 3u;   // bit 0: greater evidence gained, bit 1: smaller
+        ws.flut[i] = (uint8_t)(gain << 6 | ((res & F_G) ? 0u : 1u) | ((res & F_S) ? 0u : 2u));
+    }
+}
+
 // ---- mark + write-back of changed seam zones -----------------------------
 
 // One wave per tile, a fixed grid of waves over contiguous tile ranges (the
@@ -561,11 +581,14 @@ __global__ __launch_bounds__(256) void k_resolve(Ws ws) {
 // Tiles without an open fragment are skipped on their counts word alone.
 template <int T, bool MULTI>
 __global__ __launch_bounds__(256) void k_writeback(Images ims, Ws ws) {
-    __shared__ uint8_t lut[4][4 * T];
+    // per wave: the resolved byte of every fragment id (0 .. S - 1) and, at
+    // REC_CLOSED + c, the closed code c itself (final masks c, no change)
+    __shared__ alignas(256) uint8_t lut[4 * 256];
     __shared__ uint32_t s_dirty;
-    if (threadIdx.x == 0) s_dirty = 0;
-    __syncthreads();
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (threadIdx.x == 0) s_dirty = 0;
+    if (lane < 4) lut[wid * 256 + REC_CLOSED + lane] = (uint8_t)lane;
+    __syncthreads();
     const int64_t nw = (int64_t)gridDim.x * 4;
     const int64_t w = xcd_block(blockIdx.x, gridDim.x) * 4 + wid;
     const int64_t t0 = ws.ntiles * w / nw, t1 = ws.ntiles * (w + 1) / nw;
@@ -583,7 +606,7 @@ __global__ __launch_bounds__(256) void k_writeback(Images ims, Ws ws) {
             const int64_t t = tb + i;
             const uint32_t cw = __shfl(cwl, i, 64), b = __shfl(bl, i, 64);
             const ImgDev im = locate<MULTI>(ims, ws.tile_img, t);
-            nd += tile_writeback<T>(lut[wid], im, t, cw, b, ws) ? 1u : 0u;
+            nd += tile_writeback<T>(lut, (uint32_t)wid, im, t, cw, b, ws) ? 1u : 0u;
         }
     }
     if (lane == 0 && nd) atomicAdd(&s_dirty, nd);
@@ -733,6 +756,7 @@ static hipError_t launch_all(const Images& ims, const Ws& ws, int T, int NT, int
     const unsigned resolve_grid =
         (unsigned)std::max<int64_t>(1, std::min<int64_t>((max_frag + 255) / 256, (int64_t)grid_cap * 2));
     k_resolve<<<resolve_grid, 256, 0, s>>>(ws);
+    k_final<<<resolve_grid, 256, 0, s>>>(ws);
     switch (T) {
     case 16: launch_writeback<16, MULTI>(ims, ws, grid_cap, s); break;
     case 32: launch_writeback<32, MULTI>(ims, ws, grid_cap, s); break;

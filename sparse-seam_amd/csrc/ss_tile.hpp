@@ -937,10 +937,12 @@ __device__ __forceinline__ void tile_profile(TileSmem<T>& sm, const ImgDev& im, 
 
 // Mark and write-back of one tile by one wave (ss_device.hpp, step 5).
 // cw is the tile's counts word; only a tile with an open fragment, or a
-// uniform tile (whose one fragment may be open), can change.  Lanes resolve
-// the tile's fragments against the seam union-find into the wave's LDS table
-// (bits 6 / 7: the fragment's max / min mask changed; bits 0-1: its final max /
-// min masks).  If any changed, the wave rewrites the tile: a uniform tile is
+// uniform tile (whose one fragment may be open), can change.  Lanes copy the
+// tile's resolved fragment bytes (k_final) into the wave's 256-byte LDS table
+// lut[wid] (bits 6 / 7: the fragment's max / min mask changed; bits 0-1: its
+// final max / min masks; entries REC_CLOSED + c hold the closed code c, so a
+// record byte of either kind is one lookup).  If any changed, the wave
+// rewrites the tile: a uniform tile is
 // one fragment; any other tile streams its record plane (fragment ids, or the
 // final masks of closed pixels) and rewrites, per mask, every whole tile row
 // that holds a pixel whose value in that mask changed, so HBM sees full row
@@ -949,19 +951,17 @@ __device__ __forceinline__ void tile_profile(TileSmem<T>& sm, const ImgDev& im, 
 // tile's base (dense id of its first fragment).  Returns whether the tile
 // changed.
 template <int T>
-__device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, int64_t gtile, uint32_t cw,
-                                               uint32_t b, const Ws& ws) {
+__device__ __forceinline__ bool tile_writeback(uint8_t* lut0, uint32_t wid, const ImgDev& im, int64_t gtile,
+                                               uint32_t cw, uint32_t b, const Ws& ws) {
+    static_assert(4 * T - 4 <= (int)REC_CLOSED, "fragment ids must stay below the closed codes");
     const int lane = threadIdx.x & 63;
+    uint8_t* lut = lut0 + wid * 256;
     const uint32_t cnt = cw & COUNT_MASK;
     bool changed = false;
     for (uint32_t j = lane; j < cnt; j += 64) {
-        const uint32_t own = ws.dbits[b + j];
-        uint32_t res = own & 3u;
-        if (res != 3u) res |= (ws.dbits[gfind_ro(ws.dparent, b + j)] >> 2) & 3u;
-        // gained greater evidence: the max mask changes; smaller: the min mask
-        const uint32_t gain = res & ~own & 3u;
-        changed = changed || gain != 0;
-        lut[j] = (uint8_t)(gain << 6 | ((res & F_G) ? 0u : 1u) | ((res & F_S) ? 0u : 2u));
+        const uint32_t v = ws.flut[b + j];
+        changed = changed || (v & 0xC0u) != 0;
+        lut[j] = (uint8_t)v;
     }
     if (!__ballot(changed)) return false;
     // the table was written by other lanes of this wave: LDS operations of one
@@ -1001,41 +1001,38 @@ __device__ __forceinline__ bool tile_writeback(uint8_t* lut, const ImgDev& im, i
     const uint64_t rows = (cw & CNT_ROWS) ? ws.orows[gtile] : ~0ull;
     for (int u = lane; u < T * T / 16; u += 64) {
         const int ly = u / VR, c0 = 16 * (u % VR);
-        const uint4 rv = ((rows >> ly) & 1u) ? *reinterpret_cast<const uint4*>(plane + ly * T + c0)
-                                             : make_uint4(CL4, CL4, CL4, CL4);
+        // a row without an open pixel holds closed pixels only: unchanged
+        if (!((rows >> ly) & 1u) || ly >= th || c0 >= tw) continue;
+        const uint4 rv = *reinterpret_cast<const uint4*>(plane + ly * T + c0);
         uint32_t rw[4] = {rv.x, rv.y, rv.z, rv.w};
-        uint32_t mxw[4], mnw[4];
-        uint32_t ch = 0;   // bit 6: a max value changed, bit 7: a min value
+        if (c0 + 16 > tw) {
 #pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            if (ly >= th) {
-                rw[d] = CL4;
-            } else {
+            for (int d = 0; d < 4; ++d) {
                 const int nv = tw - c0 - 4 * d;   // valid bytes of this dword
                 if (nv < 4) rw[d] = nv <= 0 ? CL4 : (rw[d] & (0xFFFFFFFFu >> (32 - 8 * nv))) | (CL4 << (8 * nv));
             }
-            const uint32_t w = rw[d];
-            if ((w & CL4) == CL4) {
-                // four closed pixels: final masks straight from the codes
-                mxw[d] = (w & 0x01010101u) * 0xFFu;
-                mnw[d] = ((w >> 1) & 0x01010101u) * 0xFFu;
-            } else {
-                mxw[d] = 0;
-                mnw[d] = 0;
+        }
+        // one table lookup per byte: the LDS address lut0 + wid * 256 + byte
+        // is assembled by one byte permute (byte 0: the record byte, byte 1:
+        // wid); the four results are packed back into a dword
+        uint32_t mxw[4], mnw[4];
+        uint32_t ch = 0;   // bits 6 / 7 of any byte: a max / min value changed
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const uint32_t r = (w >> (8 * i)) & 0xFFu;
-                    const uint32_t v = r >= REC_CLOSED ? (r & 3u) : (uint32_t)lut[r];
-                    ch |= v;
-                    mxw[d] |= ((v & 1u) ? 0xFFu : 0u) << (8 * i);
-                    mnw[d] |= ((v & 2u) ? 0xFFu : 0u) << (8 * i);
-                }
-            }
+        for (int d = 0; d < 4; ++d) {
+            const uint32_t w = rw[d];
+            const uint32_t v = (uint32_t)lut0[__builtin_amdgcn_perm(wid, w, 0x0C0C0400u)] |
+                               (uint32_t)lut0[__builtin_amdgcn_perm(wid, w, 0x0C0C0401u)] << 8 |
+                               (uint32_t)lut0[__builtin_amdgcn_perm(wid, w, 0x0C0C0402u)] << 16 |
+                               (uint32_t)lut0[__builtin_amdgcn_perm(wid, w, 0x0C0C0403u)] << 24;
+            ch |= v;
+            const uint32_t x = v & 0x01010101u, n = v & 0x02020202u;
+            mxw[d] = (x << 8) - x;          // bit 0 of each byte -> 0x00 / 0xFF
+            mnw[d] = (n << 7) - (n >> 1);   // bit 1 of each byte -> 0x00 / 0xFF
         }
         const uint32_t rowbits = (1u << VR) - 1u, first = (uint32_t)lane & ~(VR - 1u);
-        const bool wx = (__ballot((ch & 0x40u) != 0) >> first) & rowbits;
-        const bool wn = (__ballot((ch & 0x80u) != 0) >> first) & rowbits;
-        if (!(wx || wn) || ly >= th || c0 >= tw) continue;
+        const bool wx = (__ballot((ch & 0x40404040u) != 0) >> first) & rowbits;
+        const bool wn = (__ballot((ch & 0x80808080u) != 0) >> first) & rowbits;
+        if (!(wx || wn)) continue;
         const int64_t o = (int64_t)(y0 + ly) * im.opitch + x0 + c0;
         if (c0 + 16 <= tw) {
             if (wx) store_u128(im.mx + o, make_uint4(mxw[0], mxw[1], mxw[2], mxw[3]));
