@@ -285,73 +285,79 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
     __syncthreads();
     // Inner seams: positions [0, NR) are right columns (seam s between block
     // tiles (s % (BT-1), s / (BT-1)) and the tile right of it), [NR, 2 NR)
-    // bottom rows (seam s between (s % BT, s / BT) and the tile below).  A
-    // wave covers consecutive positions of one seam, so a pair equal to the
-    // previous position's (lane - 1) is skipped.
+    // bottom rows (seam s between (s % BT, s / BT) and the tile below).
+    // Position k of a seam pairs pixel A(k) of tile i's edge with the pixels
+    // B(k - 1), B(k), B(k + 1) of the facing edge (equality bits 0, 1, 2).
+    // Most pairs are implied by another one exactly, with the same two
+    // fragments, because equal pixels adjacent along one tile edge share a
+    // fragment:
+    //   (A(k), B(k)) with bit 0 at k and bit 1 at k - 1: A(k-1) = B(k-1) =
+    //     A(k) = B(k), the pair of position k - 1;
+    //   (A(k), B(k-1)), k > 0, with bit 1 at k: B(k-1) = A(k) = B(k), the
+    //     pair (A(k), B(k)); with bit 1 at k - 1: A(k-1) = B(k-1) = A(k), the
+    //     pair of position k - 1;
+    //   (A(k), B(k+1)), k < T - 1, likewise with bit 1 at k or k + 1.
+    // So a position unites its straight pair unless the one before holds it,
+    // and a diagonal pair only at a tile corner or where neither straight
+    // pair exists.
     constexpr int NR = (MERGE_BT - 1) * MERGE_BT * T;
     bool bad = false;
     uint32_t retries = 0;
+    auto unite = [&](uint32_t a, int ta, uint32_t b, int tb) {
+        if (in_lds) lunion_pair(par, a, b);
+        else gunion(ws.dparent, gb[ta] + (a - lb[ta]), gb[tb] + (b - lb[tb]), retries);
+    };
 #pragma unroll 1
     for (int P0 = 0; P0 < 2 * NR; P0 += MERGE_THREADS) {
         const int P = P0 + threadIdx.x;
-        uint32_t a = NONE, bs[3] = {NONE, NONE, NONE};
-        uint32_t ta = 0, tb[3] = {0, 0, 0};
         const int k = P % T;
-        if (P < NR) {
-            const int sidx = P / T, ix = sidx % (MERGE_BT - 1), iy = sidx / (MERGE_BT - 1);
-            const int i = iy * MERGE_BT + ix;
-            const uint32_t ci = lb[i + 1] - lb[i];
-            const uint32_t e = (ix + 1 < ntx && iy < nty && ci) ? seq[i][k] : 0u;
-            if (e) {
-                const uint32_t ja = srec[i][3 * T + k];
-                if (ja < ci) { a = lb[i] + ja; ta = i; } else bad = true;
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    if (!(e & (1u << d)) || a == NONE) continue;
-                    const int ny = k + d - 1, dy = ny < 0 ? -1 : (ny >= T ? 1 : 0);
-                    const int niy = iy + dy;
-                    if (niy < 0 || niy >= nty) continue;   // another block's seam pair
-                    const int nt = niy * MERGE_BT + ix + 1;
-                    const uint32_t jb = srec[nt][2 * T + ny - dy * T];
-                    if (jb < lb[nt + 1] - lb[nt]) { bs[d] = lb[nt] + jb; tb[d] = nt; } else bad = true;
-                }
-            }
-        } else if (P < 2 * NR) {
-            const int sidx = (P - NR) / T, ix = sidx % MERGE_BT, iy = sidx / MERGE_BT;
-            const int i = iy * MERGE_BT + ix;
-            const uint32_t ci = lb[i + 1] - lb[i];
-            const uint32_t e = (ix < ntx && iy + 1 < nty && ci) ? seq[i][T + k] : 0u;
-            if (e) {
-                const uint32_t ja = srec[i][T + k];
-                if (ja < ci) { a = lb[i] + ja; ta = i; } else bad = true;
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    if (!(e & (1u << d)) || a == NONE) continue;
-                    const int nx = k + d - 1, dx = nx < 0 ? -1 : (nx >= T ? 1 : 0);
-                    const int nix = ix + dx;
-                    if (nix < 0 || nix >= ntx) continue;   // another block's seam pair
-                    const int nt = (iy + 1) * MERGE_BT + nix;
-                    const uint32_t jb = srec[nt][nx - dx * T];
-                    if (jb < lb[nt + 1] - lb[nt]) { bs[d] = lb[nt] + jb; tb[d] = nt; } else bad = true;
-                }
-            }
+        const bool right = P < NR;
+        const int sidx = right ? P / T : (P - NR) / T;
+        const int ix = right ? sidx % (MERGE_BT - 1) : sidx % MERGE_BT;
+        const int iy = right ? sidx / (MERGE_BT - 1) : sidx / MERGE_BT;
+        const int i = iy * MERGE_BT + ix;
+        const int nb = right ? i + 1 : i + MERGE_BT;   // the facing tile
+        uint32_t e = 0, ep = 0, en = 0;
+        if (P < 2 * NR && (right ? (ix + 1 < ntx && iy < nty) : (ix < ntx && iy + 1 < nty)) && lb[i + 1] != lb[i]) {
+            const uint8_t* eqs = seq[i] + (right ? 0 : T);
+            e = eqs[k];
+            ep = k > 0 ? eqs[k - 1] : 0u;
+            en = k < T - 1 ? eqs[k + 1] : 0u;
         }
-        const uint32_t pa = __shfl_up(a, 1, 64);
-        uint32_t pbs[3];
+        const bool s1 = (e & 2u) != 0;
+        const bool u1 = s1 && !((e & 1u) && (ep & 2u));
+        const bool u0 = (e & 1u) && (k == 0 || (!s1 && !(ep & 2u)));
+        const bool u2 = (e & 4u) && (k == T - 1 || (!s1 && !(en & 2u)));
+        if (!e) continue;
+        const uint32_t ci = lb[i + 1] - lb[i];
+        const uint32_t ja = srec[i][right ? 3 * T + k : T + k];
+        if (ja >= ci) {
+            bad = true;
+            continue;
+        }
+        const uint32_t a = lb[i] + ja;
+        if (u1) {
+            const uint32_t jb = srec[nb][right ? 2 * T + k : k];
+            if (jb < lb[nb + 1] - lb[nb]) unite(a, i, lb[nb] + jb, nb);
+            else bad = true;
+        }
 #pragma unroll
-        for (int d = 0; d < 3; ++d) pbs[d] = __shfl_up(bs[d], 1, 64);
-        const bool has_prev = lane > 0 && k > 0 && pa == a;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            if (bs[d] == NONE) continue;
-            bool dup = (d > 0 && bs[d] == bs[d - 1]) || (d > 1 && bs[d] == bs[d - 2]);
-            if (has_prev) dup = dup || bs[d] == pbs[0] || bs[d] == pbs[1] || bs[d] == pbs[2];
-            if (dup) continue;
-            if (in_lds) {
-                lunion_pair(par, a, bs[d]);
+        for (int d = 0; d < 3; d += 2) {
+            if (!(d == 0 ? u0 : u2)) continue;
+            const int nk = k + d - 1, dn = nk < 0 ? -1 : (nk >= T ? 1 : 0);
+            int nt;
+            if (right) {
+                const int niy = iy + dn;
+                if (niy < 0 || niy >= nty) continue;   // another block's seam pair
+                nt = niy * MERGE_BT + ix + 1;
             } else {
-                gunion(ws.dparent, gb[ta] + (a - lb[ta]), gb[tb[d]] + (bs[d] - lb[tb[d]]), retries);
+                const int nix = ix + dn;
+                if (nix < 0 || nix >= ntx) continue;   // another block's seam pair
+                nt = (iy + 1) * MERGE_BT + nix;
             }
+            const uint32_t jb = srec[nt][(right ? 2 * T : 0) + nk - dn * T];
+            if (jb < lb[nt + 1] - lb[nt]) unite(a, i, lb[nt] + jb, nt);
+            else bad = true;
         }
     }
     __syncthreads();
@@ -421,40 +427,63 @@ __global__ __launch_bounds__(SEAM_THREADS) void k_seam_merge(Images ims, Ws ws) 
     bool bad = false;
     constexpr uint32_t NONE = 0xFFFFFFFFu;
     uint32_t fa = NONE, fb[3] = {NONE, NONE, NONE};
+    uint32_t e = 0, cnt = 0, ja = 0, base = 0, jb1 = 0, cnt1 = 0, base1 = 0;
+    ImgDev im{};
+    int tx = 0, ty = 0;
+    bool right = false;
+    int64_t gtile = 0;
     if (unit < ws.nseam) {
-        const ImgDev im = MULTI ? ims.table[ws.seam_img[unit]] : ims.one;
+        im = MULTI ? ims.table[ws.seam_img[unit]] : ims.one;
         const uint32_t u = (uint32_t)(unit - im.seam_begin);
-        int tx, ty;
-        const bool right = seam_unit_edge(u, (uint32_t)im.seam_cols, (uint32_t)im.seam_rows,
-                                          (uint32_t)im.tiles_x, (uint32_t)im.tiles_y, tx, ty);
-        // by construction the tile right of (below) the edge exists
-        const int64_t gtile = im.tile_begin + (int64_t)ty * im.tiles_x + tx;
-        const uint32_t cnt = ws.counts[gtile] & COUNT_MASK;
-        const uint8_t* eq = ws.eq + gtile * 2 * T;
-        const uint8_t* rec = ws.rec + gtile * 4 * T;
-        const uint32_t e = cnt ? (uint32_t)eq[right ? k : T + k] : 0u;
-        if (e) {
-            const uint32_t base = ws.base[gtile];
-            const uint32_t ja = rec[right ? 3 * T + k : T + k];
-            if (ja < cnt) fa = base + ja; else bad = true;
+        right = seam_unit_edge(u, (uint32_t)im.seam_cols, (uint32_t)im.seam_rows, (uint32_t)im.tiles_x,
+                               (uint32_t)im.tiles_y, tx, ty);
+        // by construction the tile right of (below) the edge exists.  The
+        // words of both tiles are loaded together (one round trip); those of
+        // a tile without fragments are stale and masked by its count.
+        gtile = im.tile_begin + (int64_t)ty * im.tiles_x + tx;
+        const int64_t nt1 = right ? gtile + 1 : gtile + im.tiles_x;
+        cnt = ws.counts[gtile] & COUNT_MASK;
+        e = ws.eq[gtile * 2 * T + (right ? k : T + k)];
+        ja = ws.rec[gtile * 4 * T + (right ? 3 * T + k : T + k)];
+        base = ws.base[gtile];
+        jb1 = ws.rec[nt1 * 4 * T + (right ? 2 * T + k : k)];
+        cnt1 = ws.counts[nt1] & COUNT_MASK;
+        base1 = ws.base[nt1];
+        if (!cnt) e = 0;
+    }
+    // The equality bits of positions k - 1 and k + 1 of the same edge (a
+    // unit's T positions are consecutive lanes).  As in the block merge, a
+    // position unites its straight pair unless position k - 1 holds the same
+    // one, and a diagonal pair only at a tile corner or where neither
+    // straight pair exists (the others join the same two fragments).
+    uint32_t ep = __shfl_up(e, 1, 64), en = __shfl_down(e, 1, 64);
+    if (k == 0) ep = 0;
+    if (k == T - 1) en = 0;
+    const bool s1 = (e & 2u) != 0;
+    const bool use[3] = {(e & 1u) && (k == 0 || (!s1 && !(ep & 2u))), s1 && !((e & 1u) && (ep & 2u)),
+                         (e & 4u) && (k == T - 1 || (!s1 && !(en & 2u)))};
+    if (e) {
+        if (ja < cnt) fa = base + ja; else bad = true;
+        if (use[1] && fa != NONE) {
+            if (jb1 < cnt1) fb[1] = base1 + jb1; else bad = true;
+        }
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                if (!(e & (1u << d)) || fa == NONE) continue;
-                int64_t nt;
-                uint32_t slot;
-                if (right) {
-                    const int ny = ty * T + k + d - 1, nty = ny / T;
-                    nt = im.tile_begin + (int64_t)nty * im.tiles_x + tx + 1;
-                    slot = 2 * T + (ny - nty * T);   // left column of the tile to the right
-                } else {
-                    const int nx = tx * T + k + d - 1, ntx = nx / T;
-                    nt = im.tile_begin + (int64_t)(ty + 1) * im.tiles_x + ntx;
-                    slot = nx - ntx * T;             // top row of the tile below
-                }
-                const uint32_t jb = ws.rec[nt * 4 * T + slot];
-                if (jb >= (ws.counts[nt] & COUNT_MASK)) { bad = true; continue; }
-                fb[d] = ws.base[nt] + jb;
+        for (int d = 0; d < 3; d += 2) {
+            if (!use[d] || fa == NONE) continue;
+            int64_t nt;
+            uint32_t slot;
+            if (right) {
+                const int ny = ty * T + k + d - 1, nty = ny / T;
+                nt = im.tile_begin + (int64_t)nty * im.tiles_x + tx + 1;
+                slot = 2 * T + (ny - nty * T);   // left column of the tile to the right
+            } else {
+                const int nx = tx * T + k + d - 1, ntx = nx / T;
+                nt = im.tile_begin + (int64_t)(ty + 1) * im.tiles_x + ntx;
+                slot = nx - ntx * T;             // top row of the tile below
             }
+            const uint32_t jb = ws.rec[nt * 4 * T + slot];
+            if (jb >= (ws.counts[nt] & COUNT_MASK)) { bad = true; continue; }
+            fb[d] = ws.base[nt] + jb;
         }
     }
     // Unions act on block roots: after the block merge a fragment's parent is
