@@ -73,6 +73,7 @@ struct RowTileSmem {
     };
 static_assert(RT_UQ_OFF + 2 * RT_UQ * 4 <= RT_ROWS * RT_P + 16, "union queues must fit in the staged rows");
     alignas(16) uint16_t lab[RT * RT];
+    uint8_t edge[4][RT];   // pixel values of tile row 0, row th - 1, column 0, column tw - 1
     uint32_t count, open, err, zflags, nonuni, anygs, orow[2], qn[2];
 };
 
@@ -321,6 +322,18 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
     }
     if (live) row_tile_load(sm, im, x0, y0, tw, th, tid);
     __syncthreads();
+    // The edge pixels' values, for the numbering of the seam zones after the
+    // staged rows have been reused (the values beyond tw / th are not read).
+    if (live) {
+        const uint8_t* st = sm.img + 16;
+        if (h == 0) {
+            sm.edge[2][y] = st[(y + 1) * RT_P];
+            sm.edge[3][y] = st[(y + 1) * RT_P + tw - 1];
+        } else {
+            sm.edge[0][y] = st[RT_P + y];
+            sm.edge[1][y] = st[th * RT_P + y];
+        }
+    }
 
     // ---- classification ---------------------------------------------------
     const int nb = min(max(tw - 32 * h, 0), 32);   // valid columns of this half
@@ -663,13 +676,22 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         // right column).  Each finds its root (compressing its own entry to
         // it) and claims it: the first claim sets RL_ROOT and numbers the
         // root; its entry becomes RL_ROOT | id << 1 | evidence.  Other lanes
-        // meanwhile read only the evidence bits, which stay.
+        // meanwhile read only the evidence bits, which stay.  The seam bit of
+        // the stored evidence stays set only if a mask of the zone can still
+        // change: a zone at value 255 never gains greater evidence and one at
+        // 0 never smaller, so a zone with either kind of evidence or its value
+        // at that end of the range is final for that mask.
         {
             const int bl = y & 31;
             const int ry = y < 32 ? 0 : th - 1;
             const uint32_t xr = __builtin_amdgcn_readlane(rfx, 0), xb = __builtin_amdgcn_readlane(rfx, th - 1);
             const uint32_t ctb = bit(y < 32 ? xr : xb, bl) ? rnode(ry, 32 * h + bl) : 1u;
             const uint32_t cmid = (y > 0 && y < th - 1 && rfx) ? nrow + (__builtin_ctz(rfx) << RL_COL) : 1u;
+            // each candidate run's value: its pixel in the top / bottom row, or
+            // the left / right edge pixel of its row (a middle row's seam run
+            // holds column 0 or column tw - 1)
+            const uint8_t* const ev[2] = {&sm.edge[y < 32 ? 0 : 1][32 * h + bl],
+                                          &sm.edge[(cmid >> RL_COL) == 0 ? 2 : 3][y]};
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 const uint32_t pc = c == 0 ? ctb : cmid;
@@ -693,7 +715,9 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                 const uint32_t old = __hip_atomic_fetch_or(rl_word(sm.lab, x), RL_ROOT << sh, __ATOMIC_RELAXED,
                                                            __HIP_MEMORY_SCOPE_WORKGROUP) >> sh;
                 if (!(old & RL_ROOT)) {
-                    const uint32_t f = ((old >> RL_FSHIFT) & 7u) | (anygs && rl_gs(owngs, x) ? (F_G | F_S) : 0u) | F_SEAM;
+                    uint32_t f = ((old >> RL_FSHIFT) & 7u) | (anygs && rl_gs(owngs, x) ? (F_G | F_S) : 0u) | F_SEAM;
+                    const uint32_t v = *ev[c];
+                    if (((f & F_G) || v == 255u) && ((f & F_S) || v == 0u)) f &= ~F_SEAM;
                     uint32_t id = RL_NOID;
                     const uint32_t j = atomicAdd(&sm.count, 1u);
                     if (j >= (uint32_t)S) {
