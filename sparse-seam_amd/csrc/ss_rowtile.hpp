@@ -676,11 +676,14 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         // right column).  Each finds its root (compressing its own entry to
         // it) and claims it: the first claim sets RL_ROOT and numbers the
         // root; its entry becomes RL_ROOT | id << 1 | evidence.  Other lanes
-        // meanwhile read only the evidence bits, which stay.  The seam bit of
-        // the stored evidence stays set only if a mask of the zone can still
-        // change: a zone at value 255 never gains greater evidence and one at
+        // meanwhile read only the evidence bits, which stay, except the seam
+        // bit: the numbering keeps it only if a mask of the zone can still
+        // change (a zone at value 255 never gains greater evidence and one at
         // 0 never smaller, so a zone with either kind of evidence or its value
-        // at that end of the range is final for that mask.
+        // at that end of the range is final for that mask).  A run whose mask
+        // pass reads the bit before it is cleared treats its zone as open,
+        // which only costs record-plane bytes: the write-back finds nothing
+        // changed in it and rewrites the same mask values.
         {
             const int bl = y & 31;
             const int ry = y < 32 ? 0 : th - 1;
