@@ -187,7 +187,6 @@ __device__ __forceinline__ int block_tile_of(const uint32_t* lb, uint32_t l) {
 template <int T, bool MULTI>
 __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws) {
     constexpr int S = 4 * T - 4, NB = MERGE_BT * MERGE_BT, CAP = MergeCap<T>::value;
-    constexpr uint32_t NONE = 0xFFFFFFFFu;
     static_assert(NB == 64, "block tile scan is one wave");
     __shared__ uint32_t par[CAP];
     __shared__ uint32_t lb[NB + 1];
@@ -220,11 +219,38 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
     const uint32_t L = lb[NB];
     if (L == 0) return;
     const bool in_lds = L <= (uint32_t)CAP;
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // Block tile of each local fragment (blocks held in LDS), one byte each,
+    // in the perimeter-record array while that is not loaded: filled here for
+    // the fragment init and again for the dense write-out.
+    static_assert(CAP <= NB * 4 * T, "the tile map must fit in the record array");
+    uint8_t* const tof = &srec[0][0];
+    auto fill_tof = [&]() {
+        for (int i = wave; i < NB; i += MERGE_THREADS / 64)
+            for (uint32_t l = lb[i] + lane; l < lb[i + 1]; l += 64) tof[l] = (uint8_t)i;
+    };
+    // Perimeter records (4T bytes) and cross-seam bits (2T bytes) of the
+    // block's tiles, 16 bytes per load, all loads of a thread issued together
+    // (before the fragment init, so that they are in flight meanwhile).
+    constexpr int RV = T / 4, EV = T / 8;   // 16-byte vectors per tile
+    constexpr int NV = NB * (RV + EV);
+    constexpr int PER = (NV + MERGE_THREADS - 1) / MERGE_THREADS;
+    uint4 v[PER];
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        const int w = threadIdx.x + m * MERGE_THREADS;
+        v[m] = make_uint4(0, 0, 0, 0);
+        if (w >= NV) continue;
+        const int i = w / (RV + EV), d = w % (RV + EV);
+        if (lb[i + 1] == lb[i]) continue;
+        v[m] = d < RV ? reinterpret_cast<const uint4*>(ws.rec + gtile(i) * 4 * T)[d]
+                      : reinterpret_cast<const uint4*>(ws.eq + gtile(i) * 2 * T)[d - RV];
+    }
     for (uint32_t w = threadIdx.x; w < CAP / 16; w += MERGE_THREADS) {
         own2[w] = 0;
         acc2[w] = 0;
     }
+    if (in_lds) fill_tof();
     __syncthreads();
     // Fragment init, four fragments per thread and pass so that their
     // evidence loads are in flight together.
@@ -235,7 +261,7 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
             const uint32_t l = l0 + u * MERGE_THREADS + threadIdx.x;
             sb[u] = 0;
             if (l < L) {
-                const int t = block_tile_of(lb, l);
+                const int t = in_lds ? (int)tof[l] : block_tile_of(lb, l);
                 sb[u] = ws.sbits[gtile(t) * S + (l - lb[t])] & 3u;
             }
         }
@@ -256,31 +282,14 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
             }
         }
     }
-    // Perimeter records (4T bytes) and cross-seam bits (2T bytes) of the
-    // block's tiles, 16 bytes per load, all loads of a thread issued together.
-    {
-        constexpr int RV = T / 4, EV = T / 8;   // 16-byte vectors per tile
-        constexpr int NV = NB * (RV + EV);
-        constexpr int PER = (NV + MERGE_THREADS - 1) / MERGE_THREADS;
-        uint4 v[PER];
+    __syncthreads();   // the tile map is read no more
 #pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const int w = threadIdx.x + m * MERGE_THREADS;
-            v[m] = make_uint4(0, 0, 0, 0);
-            if (w >= NV) continue;
-            const int i = w / (RV + EV), d = w % (RV + EV);
-            if (lb[i + 1] == lb[i]) continue;
-            v[m] = d < RV ? reinterpret_cast<const uint4*>(ws.rec + gtile(i) * 4 * T)[d]
-                          : reinterpret_cast<const uint4*>(ws.eq + gtile(i) * 2 * T)[d - RV];
-        }
-#pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const int w = threadIdx.x + m * MERGE_THREADS;
-            if (w >= NV) continue;
-            const int i = w / (RV + EV), d = w % (RV + EV);
-            if (d < RV) reinterpret_cast<uint4*>(srec[i])[d] = v[m];
-            else reinterpret_cast<uint4*>(seq[i])[d - RV] = v[m];
-        }
+    for (int m = 0; m < PER; ++m) {
+        const int w = threadIdx.x + m * MERGE_THREADS;
+        if (w >= NV) continue;
+        const int i = w / (RV + EV), d = w % (RV + EV);
+        if (d < RV) reinterpret_cast<uint4*>(srec[i])[d] = v[m];
+        else reinterpret_cast<uint4*>(seq[i])[d - RV] = v[m];
     }
     __syncthreads();
     // Inner seams: positions [0, NR) are right columns (seam s between block
@@ -316,7 +325,6 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
         const int ix = right ? sidx % (MERGE_BT - 1) : sidx % MERGE_BT;
         const int iy = right ? sidx / (MERGE_BT - 1) : sidx / MERGE_BT;
         const int i = iy * MERGE_BT + ix;
-        const int nb = right ? i + 1 : i + MERGE_BT;   // the facing tile
         uint32_t e = 0, ep = 0, en = 0;
         if (P < 2 * NR && (right ? (ix + 1 < ntx && iy < nty) : (ix < ntx && iy + 1 < nty)) && lb[i + 1] != lb[i]) {
             const uint8_t* eqs = seq[i] + (right ? 0 : T);
@@ -336,25 +344,15 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
             continue;
         }
         const uint32_t a = lb[i] + ja;
-        if (u1) {
-            const uint32_t jb = srec[nb][right ? 2 * T + k : k];
-            if (jb < lb[nb + 1] - lb[nb]) unite(a, i, lb[nb] + jb, nb);
-            else bad = true;
-        }
-#pragma unroll
-        for (int d = 0; d < 3; d += 2) {
-            if (!(d == 0 ? u0 : u2)) continue;
+        // one union site for the (up to three) pairs of the position
+        uint32_t todo = (u0 ? 1u : 0u) | (u1 ? 2u : 0u) | (u2 ? 4u : 0u);
+        while (todo) {
+            const int d = __builtin_ctz(todo);
+            todo &= todo - 1u;
             const int nk = k + d - 1, dn = nk < 0 ? -1 : (nk >= T ? 1 : 0);
-            int nt;
-            if (right) {
-                const int niy = iy + dn;
-                if (niy < 0 || niy >= nty) continue;   // another block's seam pair
-                nt = niy * MERGE_BT + ix + 1;
-            } else {
-                const int nix = ix + dn;
-                if (nix < 0 || nix >= ntx) continue;   // another block's seam pair
-                nt = (iy + 1) * MERGE_BT + nix;
-            }
+            const int niy = right ? iy + dn : iy + 1, nix = right ? ix + 1 : ix + dn;
+            if (niy < 0 || niy >= nty || nix < 0 || nix >= ntx) continue;   // another block's seam pair
+            const int nt = niy * MERGE_BT + nix;
             const uint32_t jb = srec[nt][(right ? 2 * T : 0) + nk - dn * T];
             if (jb < lb[nt + 1] - lb[nt]) unite(a, i, lb[nt] + jb, nt);
             else bad = true;
@@ -362,6 +360,7 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
     }
     __syncthreads();
     if (in_lds) {
+        fill_tof();   // the records are read no more
         // evidence of each block component at its root; labels compressed
         for (uint32_t l = threadIdx.x; l < L; l += MERGE_THREADS) {
             const uint32_t r = lfind(par, l);
@@ -374,10 +373,10 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
         // dense arrays: parent = block root; a block root also keeps its
         // component's evidence in bits 2-3
         for (uint32_t l = threadIdx.x; l < L; l += MERGE_THREADS) {
-            const int t = block_tile_of(lb, l);
+            const int t = tof[l];
             const uint32_t g = gb[t] + (l - lb[t]);
             const uint32_t r = par[l];
-            const int tr = r >= lb[t] ? t : block_tile_of(lb, r);
+            const int tr = tof[r];
             ws.dparent[g] = gb[tr] + (r - lb[tr]);
             uint32_t d = (own2[l >> 4] >> (2 * (l & 15u))) & 3u;
             if (r == l) d |= (((acc2[l >> 4] >> (2 * (l & 15u))) & 3u) << 2) | DB_BROOT;
