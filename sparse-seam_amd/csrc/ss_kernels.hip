@@ -5,7 +5,7 @@
 //   tile-local Komura labels        :194-289  -> tile_label()
 //   classify / number / emit        :291-475  -> tile_stage()
 //   seam merge (3 pairs per seam px):508-581  -> k_seam_merge
-//   compress + reduce               :583-619  -> k_resolve
+//   compress + reduce               :583-619  -> k_resolve, k_final
 //   materialise seam records        :477-506  -> k_writeback
 // The reference writes a 1-byte record for every pixel and re-reads it for
 // every pixel; here masks are written once (provisionally for seam zones), the
