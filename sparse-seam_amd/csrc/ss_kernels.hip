@@ -195,6 +195,9 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
     __shared__ alignas(16) uint8_t seq[NB][2 * T];
     __shared__ uint32_t own2[CAP / 16];   // 2-bit local evidence per fragment
     __shared__ uint32_t acc2[CAP / 16];   // 2-bit OR over the component, at its root
+    constexpr int NRS = (MERGE_BT - 1) * MERGE_BT;   // right-edge inner seams (bottom likewise)
+    __shared__ uint8_t slist[2 * NRS];    // inner seams to visit
+    __shared__ uint32_t s_nseam;
     const ImgDev im = MULTI ? ims.table[ws.block_img[blockIdx.x]] : ims.one;
     const uint32_t bl = (uint32_t)((int64_t)blockIdx.x - im.block_begin);
     const int tx0 = (int)(bl % (uint32_t)im.blocks_x) * MERGE_BT, ty0 = (int)(bl / (uint32_t)im.blocks_x) * MERGE_BT;
@@ -283,6 +286,26 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
         }
     }
     __syncthreads();   // the tile map is read no more
+    if (wave == 0) {
+        // list of the inner seams to visit: slots [0, NRS) right edges, [NRS,
+        // 2 NRS) bottom edges (as in the loop below)
+        uint32_t nsl = 0;
+        for (int s0 = 0; s0 < 2 * NRS; s0 += 64) {
+            const int sl = s0 + lane;
+            bool ok = false;
+            if (sl < 2 * NRS) {
+                const bool r = sl < NRS;
+                const int q = r ? sl : sl - NRS;
+                const int ix = r ? q % (MERGE_BT - 1) : q % MERGE_BT, iy = r ? q / (MERGE_BT - 1) : q / MERGE_BT;
+                const int i = iy * MERGE_BT + ix;
+                ok = (r ? (ix + 1 < ntx && iy < nty) : (ix < ntx && iy + 1 < nty)) && lb[i + 1] != lb[i];
+            }
+            const uint64_t m = __ballot(ok);
+            if (ok) slist[nsl + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = (uint8_t)sl;
+            nsl += (uint32_t)__builtin_popcountll(m);
+        }
+        if (lane == 0) s_nseam = nsl;
+    }
 #pragma unroll
     for (int m = 0; m < PER; ++m) {
         const int w = threadIdx.x + m * MERGE_THREADS;
@@ -292,9 +315,10 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
         else reinterpret_cast<uint4*>(seq[i])[d - RV] = v[m];
     }
     __syncthreads();
-    // Inner seams: positions [0, NR) are right columns (seam s between block
-    // tiles (s % (BT-1), s / (BT-1)) and the tile right of it), [NR, 2 NR)
-    // bottom rows (seam s between (s % BT, s / BT) and the tile below).
+    // Inner seams: slots s in [0, NRS) are right columns (between block tile
+    // (s % (BT-1), s / (BT-1)) and the tile right of it), s in [NRS, 2 NRS)
+    // bottom rows (between tile ((s - NRS) % BT, (s - NRS) / BT) and the tile
+    // below); T positions per seam.
     // Position k of a seam pairs pixel A(k) of tile i's edge with the pixels
     // B(k - 1), B(k), B(k + 1) of the facing edge (equality bits 0, 1, 2).
     // Most pairs are implied by another one exactly, with the same two
@@ -309,24 +333,28 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
     // So a position unites its straight pair unless the one before holds it,
     // and a diagonal pair only at a tile corner or where neither straight
     // pair exists.
-    constexpr int NR = (MERGE_BT - 1) * MERGE_BT * T;
     bool bad = false;
     uint32_t retries = 0;
     auto unite = [&](uint32_t a, int ta, uint32_t b, int tb) {
         if (in_lds) lunion_pair(par, a, b);
         else gunion(ws.dparent, gb[ta] + (a - lb[ta]), gb[tb] + (b - lb[tb]), retries);
     };
+    // Only the inner seams that exist (a block at an image's right or bottom
+    // edge may be partial) and whose first tile holds a fragment are visited:
+    // their slots, in order, were listed by wave 0.
+    const int NP = (int)s_nseam * T;
 #pragma unroll 1
-    for (int P0 = 0; P0 < 2 * NR; P0 += MERGE_THREADS) {
+    for (int P0 = 0; P0 < NP; P0 += MERGE_THREADS) {
         const int P = P0 + threadIdx.x;
         const int k = P % T;
-        const bool right = P < NR;
-        const int sidx = right ? P / T : (P - NR) / T;
+        const int sl = P < NP ? (int)slist[P / T] : 0;
+        const bool right = sl < NRS;
+        const int sidx = right ? sl : sl - NRS;
         const int ix = right ? sidx % (MERGE_BT - 1) : sidx % MERGE_BT;
         const int iy = right ? sidx / (MERGE_BT - 1) : sidx / MERGE_BT;
         const int i = iy * MERGE_BT + ix;
         uint32_t e = 0, ep = 0, en = 0;
-        if (P < 2 * NR && (right ? (ix + 1 < ntx && iy < nty) : (ix < ntx && iy + 1 < nty)) && lb[i + 1] != lb[i]) {
+        if (P < NP) {
             const uint8_t* eqs = seq[i] + (right ? 0 : T);
             e = eqs[k];
             ep = k > 0 ? eqs[k - 1] : 0u;
