@@ -415,6 +415,215 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
     if (bad) atomicOr(&ws.hdr->error, 2u);
 }
 
+// ---- pairs across a tile edge (group and seam merges) ---------------------
+
+// Words of position k of a tile edge (right edge of tile gtile if `right`,
+// else its bottom edge) and of the facing tile nt1, loaded together (one round
+// trip); those of a tile without fragments are stale and masked by its count.
+struct EdgeWords {
+    uint32_t e = 0, cnt = 0, ja = 0, base = 0, jb1 = 0, cnt1 = 0, base1 = 0;
+};
+template <int T>
+__device__ __forceinline__ EdgeWords load_edge(const Ws& ws, int64_t gtile, int64_t nt1, bool right, int k) {
+    EdgeWords w;
+    w.cnt = ws.counts[gtile] & COUNT_MASK;
+    w.e = ws.eq[gtile * 2 * T + (right ? k : T + k)];
+    w.ja = ws.rec[gtile * 4 * T + (right ? 3 * T + k : T + k)];
+    w.base = ws.base[gtile];
+    w.jb1 = ws.rec[nt1 * 4 * T + (right ? 2 * T + k : k)];
+    w.cnt1 = ws.counts[nt1] & COUNT_MASK;
+    w.base1 = ws.base[nt1];
+    if (!w.cnt) w.e = 0;
+    return w;
+}
+
+// Which of the three pairs (A(k), B(k - 1 + d)) of an edge position to unite.
+// Equal pixels adjacent along one tile edge share a fragment, so most pairs
+// join the same two fragments as another one: the straight pair is skipped
+// when position k - 1 holds the same one (bit 0 at k and bit 1 at k - 1), a
+// diagonal pair whenever this position or the neighbouring one has a straight
+// pair, except at a tile corner.  A unit's T positions are consecutive lanes
+// (every lane of the wave calls this: it reads the neighbours' bits).
+template <int T>
+__device__ __forceinline__ void edge_pairs_used(uint32_t e, int k, bool use[3]) {
+    uint32_t ep = __shfl_up(e, 1, 64), en = __shfl_down(e, 1, 64);
+    if (k == 0) ep = 0;
+    if (k == T - 1) en = 0;
+    const bool s1 = (e & 2u) != 0;
+    use[0] = (e & 1u) && (k == 0 || (!s1 && !(ep & 2u)));
+    use[1] = s1 && !((e & 1u) && (ep & 2u));
+    use[2] = (e & 4u) && (k == T - 1 || (!s1 && !(en & 2u)));
+}
+
+// Block roots of the used pairs: ra for A(k), rb[d] for B(k - 1 + d) (NONE if
+// unused).  A perimeter record out of range (a broken tile stage) sets bad.
+template <int T>
+__device__ __forceinline__ void edge_pair_roots(const Ws& ws, const ImgDev& im, const EdgeWords& w, int tx, int ty,
+                                                bool right, int k, const bool use[3], uint32_t& ra, uint32_t rb[3],
+                                                bool& bad) {
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    uint32_t fa = NONE, fb[3] = {NONE, NONE, NONE};
+    ra = NONE;
+    rb[0] = rb[1] = rb[2] = NONE;
+    if (!w.e) return;
+    if (w.ja < w.cnt) fa = w.base + w.ja; else bad = true;
+    if (use[1] && fa != NONE) {
+        if (w.jb1 < w.cnt1) fb[1] = w.base1 + w.jb1; else bad = true;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (d == 1 || !use[d] || fa == NONE) continue;   // the diagonals
+        int64_t nt;
+        uint32_t slot;
+        if (right) {
+            const int ny = ty * T + k + d - 1, nty = ny / T;
+            nt = im.tile_begin + (int64_t)nty * im.tiles_x + tx + 1;
+            slot = 2 * T + (ny - nty * T);   // left column of the tile to the right
+        } else {
+            const int nx = tx * T + k + d - 1, ntx = nx / T;
+            nt = im.tile_begin + (int64_t)(ty + 1) * im.tiles_x + ntx;
+            slot = nx - ntx * T;             // top row of the tile below
+        }
+        const uint32_t jb = ws.rec[nt * 4 * T + slot];
+        if (jb >= (ws.counts[nt] & COUNT_MASK)) { bad = true; continue; }
+        fb[d] = ws.base[nt] + jb;
+    }
+    // after the block merge a fragment's parent is its block root (or, once
+    // that root is linked, an ancestor of it)
+    if (fa != NONE) {
+        ra = ws.dparent[fa];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (fb[d] != NONE) rb[d] = ws.dparent[fb[d]];
+    }
+}
+
+// ---- group merge: the block seams inside 2 x 2 groups of merge blocks -----
+
+// Every union of the global seam merge ends in a device-scope CAS, and their
+// number, not the walks, sets that kernel's time (with plain stores in place
+// of the CAS it ran in 40 % of the time).  Merge blocks are paired up: in each
+// group of 2 x 2 blocks (GROUP_T x GROUP_T tiles) the two block seams inside
+// it are merged here, in LDS, by one workgroup that alone touches the group's
+// block roots, so its links are plain stores; the seam merge then takes only
+// the seams between groups (and the diagonal contacts that leave a group at a
+// tile corner).  The workgroup of the group's top-left block does the work
+// (the others exit): its edges are the right edges of the tiles in the last
+// tile column of its block column (vertical seam, GROUP_T tiles) and the
+// bottom edges of the tiles in the last tile row of its block row (horizontal
+// seam), T positions each.  The group's distinct block roots get LDS slots in
+// an open-addressed hash table and are united there; then each joins the
+// smallest id of its component (a smaller id, as the seam merge's links).  A
+// group with more roots than the table holds unites all its pairs with the
+// seam merge's CAS instead.
+constexpr int GROUP_T = 2 * MERGE_BT;   // tiles per group side
+constexpr int GROUP_THREADS = 1024;
+constexpr int GSET = 4096;              // LDS slots for the distinct block roots of a group
+
+template <int T, bool MULTI>
+__global__ __launch_bounds__(GROUP_THREADS) void k_group_merge(Images ims, Ws ws) {
+    constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+    constexpr int NPOS = 2 * GROUP_T * T;             // edge positions of a group
+    constexpr int PER = (NPOS + GROUP_THREADS - 1) / GROUP_THREADS;   // per thread
+    static_assert(GROUP_THREADS % T == 0, "whole edges per thread pass");
+    __shared__ uint32_t key[GSET];    // block-root id of a slot
+    __shared__ uint32_t gpar[GSET];   // union-find over slots
+    __shared__ uint32_t gmin[GSET];   // smallest id of a component, at its root slot
+    __shared__ uint32_t used[GSET];   // the slots in use
+    __shared__ uint32_t s_over, s_used;
+    const ImgDev im = MULTI ? ims.table[ws.block_img[blockIdx.x]] : ims.one;
+    const uint32_t bl = (uint32_t)((int64_t)blockIdx.x - im.block_begin);
+    const int bx = (int)(bl % (uint32_t)im.blocks_x), by = (int)(bl / (uint32_t)im.blocks_x);
+    if ((bx | by) & 1) return;
+    const bool hasV = bx + 1 < im.blocks_x, hasH = by + 1 < im.blocks_y;
+    if (!hasV && !hasH) return;
+    for (int i = threadIdx.x; i < GSET; i += GROUP_THREADS) {
+        key[i] = EMPTY;
+        gpar[i] = (uint32_t)i;
+        gmin[i] = EMPTY;
+    }
+    if (threadIdx.x == 0) {
+        s_over = 0;
+        s_used = 0;
+    }
+    bool bad = false;
+    uint32_t ra[PER], rb[PER][3];
+    bool keep[PER][3];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int p = (int)threadIdx.x + q * GROUP_THREADS;
+        const int edge = p / T, k = p % T;
+        const bool right = edge < GROUP_T;
+        const int tx = right ? MERGE_BT * bx + MERGE_BT - 1 : MERGE_BT * bx + (edge - GROUP_T);
+        const int ty = right ? MERGE_BT * by + edge : MERGE_BT * by + MERGE_BT - 1;
+        const bool valid = p < NPOS && (right ? (hasV && ty < im.tiles_y) : (hasH && tx < im.tiles_x));
+        EdgeWords w;
+        const int64_t gtile = im.tile_begin + (int64_t)ty * im.tiles_x + tx;
+        if (valid) w = load_edge<T>(ws, gtile, right ? gtile + 1 : gtile + im.tiles_x, right, k);
+        bool use[3];
+        edge_pairs_used<T>(w.e, k, use);
+        // a diagonal pair at a tile corner may leave the group: the seam
+        // merge takes it
+        const int along = right ? ty : tx, lim = right ? im.tiles_y : im.tiles_x;
+        const int first = right ? MERGE_BT * by : MERGE_BT * bx;
+        if (k == 0 && along - 1 < first) use[0] = false;
+        if (k == T - 1 && (along + 1 >= first + GROUP_T || along + 1 >= lim)) use[2] = false;
+        edge_pair_roots<T>(ws, im, w, tx, ty, right, k, use, ra[q], rb[q], bad);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) keep[q][d] = rb[q][d] != 0xFFFFFFFFu;
+    }
+    __syncthreads();   // table initialised
+    auto slot_of = [&](uint32_t id) -> int {
+        const uint32_t h = (id * 0x9E3779B1u) >> 20;
+        static_assert(GSET == 4096, "the hash above yields 12 bits");
+        for (int pr = 0; pr < 64; ++pr) {
+            const int sl = (int)((h + (uint32_t)pr) & (GSET - 1));
+            const uint32_t old = atomicCAS(&key[sl], EMPTY, id);
+            if (old == EMPTY) used[atomicAdd(&s_used, 1u)] = (uint32_t)sl;
+            if (old == EMPTY || old == id) return sl;
+        }
+        return -1;
+    };
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (!keep[q][d]) continue;
+            const int sa = slot_of(ra[q]), sb = slot_of(rb[q][d]);
+            if (sa < 0 || sb < 0) {
+                s_over = 1u;
+                continue;
+            }
+            lunion_pair(gpar, (uint32_t)sa, (uint32_t)sb);
+        }
+    }
+    __syncthreads();
+    const uint32_t nused = s_used;
+    for (uint32_t u = threadIdx.x; u < nused; u += GROUP_THREADS) {
+        const uint32_t i = used[u];
+        atomicMin(&gmin[lfind(gpar, i)], key[i]);
+    }
+    __syncthreads();
+    for (uint32_t u = threadIdx.x; u < nused; u += GROUP_THREADS) {
+        const uint32_t i = used[u], id = key[i];
+        const uint32_t m = gmin[lfind(gpar, i)];
+        if (m != id) ws.dparent[id] = m;   // id was a block root: no one else writes it meanwhile
+    }
+    uint32_t retries = 0;
+    if (s_over) {
+        // the table overflowed: every pair of the group through the CAS union
+        __threadfence();
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                if (keep[q][d]) gunion(ws.dparent, ra[q], rb[q][d], retries);
+    }
+    if (retries) atomicAdd((unsigned long long*)&ws.hdr->retries, (unsigned long long)retries);
+    if (bad) atomicOr(&ws.hdr->error, 2u);
+}
+
 // ---- seam merge -----------------------------------------------------------
 
 // One thread per (seam unit, k).  A unit is a tile edge on a merge-block
@@ -427,16 +636,18 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
 // corner is both the up-right contact of one tile's right edge and the
 // down-left contact of the bottom edge of the tile above-right of it (resp.
 // the down-right contact of both edges of one tile).  Pairs inside one block
-// were united by k_block_merge.
+// were united by k_block_merge, pairs inside a 2 x 2 group of blocks by
+// k_group_merge: of a seam inside a group only the diagonal pairs at a tile
+// corner that reach another group are left.
 //
 // A plateau crossing a block seam yields the same pair of block components at
 // many pixels of the seam, and every union ends in a device-scope CAS (or
 // several, when they collide), which is what this kernel's time goes to.  So
 // the units of one block seam are consecutive (a workgroup of SEAM_THREADS
 // covers the MERGE_BT edges of one seam at T = 64), and each distinct pair of
-// block roots is united once per workgroup: the wave first keeps one lane per
-// distinct pair, then an LDS hash set drops the pairs another wave of the
-// workgroup already holds.
+// block roots is united once per workgroup: an LDS hash set drops the pairs
+// another lane of the workgroup already holds (a wave-level leader loop that
+// found the distinct pairs first cost more than it saved).
 constexpr int SEAM_THREADS = 512;
 constexpr int SEAM_SET = 2048;   // LDS hash-set slots (pairs a workgroup may keep: 3 per thread)
 
@@ -449,102 +660,43 @@ __global__ __launch_bounds__(SEAM_THREADS) void k_seam_merge(Images ims, Ws ws) 
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t unit = g / T;
     const int k = (int)(g % T);
-    const int lane = threadIdx.x & 63;
     uint32_t retries = 0;
     bool bad = false;
-    constexpr uint32_t NONE = 0xFFFFFFFFu;
-    uint32_t fa = NONE, fb[3] = {NONE, NONE, NONE};
-    uint32_t e = 0, cnt = 0, ja = 0, base = 0, jb1 = 0, cnt1 = 0, base1 = 0;
+    EdgeWords w;
     ImgDev im{};
     int tx = 0, ty = 0;
-    bool right = false;
-    int64_t gtile = 0;
+    bool right = false, inner = false;
+    int along = 0;
     if (unit < ws.nseam) {
         im = MULTI ? ims.table[ws.seam_img[unit]] : ims.one;
         const uint32_t u = (uint32_t)(unit - im.seam_begin);
         right = seam_unit_edge(u, (uint32_t)im.seam_cols, (uint32_t)im.seam_rows, (uint32_t)im.tiles_x,
                                (uint32_t)im.tiles_y, tx, ty);
-        // by construction the tile right of (below) the edge exists.  The
-        // words of both tiles are loaded together (one round trip); those of
-        // a tile without fragments are stale and masked by its count.
-        gtile = im.tile_begin + (int64_t)ty * im.tiles_x + tx;
-        const int64_t nt1 = right ? gtile + 1 : gtile + im.tiles_x;
-        cnt = ws.counts[gtile] & COUNT_MASK;
-        e = ws.eq[gtile * 2 * T + (right ? k : T + k)];
-        ja = ws.rec[gtile * 4 * T + (right ? 3 * T + k : T + k)];
-        base = ws.base[gtile];
-        jb1 = ws.rec[nt1 * 4 * T + (right ? 2 * T + k : k)];
-        cnt1 = ws.counts[nt1] & COUNT_MASK;
-        base1 = ws.base[nt1];
-        if (!cnt) e = 0;
+        // a seam inside a group: between block columns (rows) 2 i and 2 i + 1
+        inner = (((right ? tx : ty) / MERGE_BT) & 1) == 0;
+        along = right ? ty : tx;
+        const bool need = !inner || (k == 0 && along % GROUP_T == 0) || (k == T - 1 && (along + 1) % GROUP_T == 0);
+        // by construction the tile right of (below) the edge exists
+        const int64_t gtile = im.tile_begin + (int64_t)ty * im.tiles_x + tx;
+        if (need) w = load_edge<T>(ws, gtile, right ? gtile + 1 : gtile + im.tiles_x, right, k);
     }
-    // The equality bits of positions k - 1 and k + 1 of the same edge (a
-    // unit's T positions are consecutive lanes).  As in the block merge, a
-    // position unites its straight pair unless position k - 1 holds the same
-    // one, and a diagonal pair only at a tile corner or where neither
-    // straight pair exists (the others join the same two fragments).
-    uint32_t ep = __shfl_up(e, 1, 64), en = __shfl_down(e, 1, 64);
-    if (k == 0) ep = 0;
-    if (k == T - 1) en = 0;
-    const bool s1 = (e & 2u) != 0;
-    const bool use[3] = {(e & 1u) && (k == 0 || (!s1 && !(ep & 2u))), s1 && !((e & 1u) && (ep & 2u)),
-                         (e & 4u) && (k == T - 1 || (!s1 && !(en & 2u)))};
-    if (e) {
-        if (ja < cnt) fa = base + ja; else bad = true;
-        if (use[1] && fa != NONE) {
-            if (jb1 < cnt1) fb[1] = base1 + jb1; else bad = true;
-        }
-#pragma unroll
-        for (int d = 0; d < 3; d += 2) {
-            if (!use[d] || fa == NONE) continue;
-            int64_t nt;
-            uint32_t slot;
-            if (right) {
-                const int ny = ty * T + k + d - 1, nty = ny / T;
-                nt = im.tile_begin + (int64_t)nty * im.tiles_x + tx + 1;
-                slot = 2 * T + (ny - nty * T);   // left column of the tile to the right
-            } else {
-                const int nx = tx * T + k + d - 1, ntx = nx / T;
-                nt = im.tile_begin + (int64_t)(ty + 1) * im.tiles_x + ntx;
-                slot = nx - ntx * T;             // top row of the tile below
-            }
-            const uint32_t jb = ws.rec[nt * 4 * T + slot];
-            if (jb >= (ws.counts[nt] & COUNT_MASK)) { bad = true; continue; }
-            fb[d] = ws.base[nt] + jb;
-        }
+    bool use[3];
+    edge_pairs_used<T>(w.e, k, use);
+    if (inner) {
+        use[1] = false;
+        use[0] = use[0] && k == 0 && along % GROUP_T == 0;
+        use[2] = use[2] && k == T - 1 && (along + 1) % GROUP_T == 0;
     }
-    // Unions act on block roots: after the block merge a fragment's parent is
-    // its block root (or, once this kernel has linked that root, an ancestor
-    // of it), so the contacts of an edge collapse to the few pairs of block
-    // components that meet across it.  The wave keeps each distinct pair once
-    // (one leader per pair and round).
-    uint32_t ra = NONE, rb[3] = {NONE, NONE, NONE};
-    if (fa != NONE) {
-        ra = ws.dparent[fa];
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-            if (fb[d] != NONE) rb[d] = ws.dparent[fb[d]];
-    }
-    bool live[3], keep[3] = {false, false, false};
+    uint32_t ra, rb[3];
+    edge_pair_roots<T>(ws, im, w, tx, ty, right, k, use, ra, rb, bad);
+    bool keep[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
-        live[d] = rb[d] != NONE && !(d > 0 && rb[d] == rb[d - 1]) && !(d > 1 && rb[d] == rb[d - 2]);
-    for (;;) {
-        const uint64_t m0 = __ballot(live[0]), m1 = __ballot(live[1]), m2 = __ballot(live[2]);
-        if (!(m0 | m1 | m2)) break;
-        const int dl = m0 ? 0 : (m1 ? 1 : 2);
-        const int leader = __ffsll((unsigned long long)(m0 ? m0 : (m1 ? m1 : m2))) - 1;
-        const uint32_t la = __shfl(ra, leader, 64);
-        const uint32_t lb = __shfl(dl == 0 ? rb[0] : (dl == 1 ? rb[1] : rb[2]), leader, 64);
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            if (d == dl && lane == leader) keep[d] = true;
-            if (live[d] && ra == la && rb[d] == lb) live[d] = false;   // clears the leader's pair too
-        }
-    }
-    // Workgroup set: a pair found there was kept by another wave.  A pair
-    // that finds no slot within 8 probes is simply united (a repeated union
-    // is harmless).
+        keep[d] = rb[d] != 0xFFFFFFFFu && !(d > 0 && rb[d] == rb[d - 1]) && !(d > 1 && rb[d] == rb[d - 2]);
+    // Workgroup set: each distinct pair of block roots is united once per
+    // workgroup (a pair found there was kept by another lane).  A pair that
+    // finds no slot within 8 probes is simply united (a repeated union is
+    // harmless).
     static_assert(SEAM_SET == 2048, "the hash below yields 11 bits");
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -562,7 +714,7 @@ __global__ __launch_bounds__(SEAM_THREADS) void k_seam_merge(Images ims, Ws ws) 
     for (int d = 0; d < 3; ++d)
         if (keep[d]) gunion(ws.dparent, ra, rb[d], retries);
     const uint32_t r = wave_sum(retries);
-    if (lane == 0 && r) atomicAdd((unsigned long long*)&ws.hdr->retries, (unsigned long long)r);
+    if ((threadIdx.x & 63) == 0 && r) atomicAdd((unsigned long long*)&ws.hdr->retries, (unsigned long long)r);
     // a perimeter record out of range would mean a broken tile stage: report
     // it instead of following it
     if (bad) atomicOr(&ws.hdr->error, 2u);
@@ -765,6 +917,7 @@ static void launch_writeback(const Images& ims, const Ws& ws, int grid_cap, hipS
 template <int T, bool MULTI>
 static void launch_merge(const Images& ims, const Ws& ws, hipStream_t s) {
     k_block_merge<T, MULTI><<<(unsigned)ws.nblk, MERGE_THREADS, 0, s>>>(ims, ws);
+    k_group_merge<T, MULTI><<<(unsigned)ws.nblk, GROUP_THREADS, 0, s>>>(ims, ws);
     const int64_t merge_threads = ws.nseam * T;
     if (merge_threads > 0) {
         const unsigned merge_grid = (unsigned)((merge_threads + SEAM_THREADS - 1) / SEAM_THREADS);
