@@ -15,8 +15,12 @@
 //   3. seam merge   a) per block of 8 x 8 tiles: union-find in LDS over the
 //                   seams inside the block; every fragment's dense parent
 //                   becomes its block-local root (dense arrays initialised);
-//                   b) lock-free min-root CAS union with path halving over the
-//                   dense ids for the seams between blocks, one thread per
+//                   b) per group of 2 x 2 blocks: union-find in LDS over the
+//                   block roots that meet across the two block seams inside
+//                   the group, linked with plain stores (one workgroup owns
+//                   the group's roots);
+//                   c) lock-free min-root CAS union with path halving over the
+//                   dense ids for the seams between groups, one thread per
 //                   pixel of a tile edge on a block boundary; the edges of a
 //                   block seam are consecutive and a workgroup unites each
 //                   distinct pair of block roots once.

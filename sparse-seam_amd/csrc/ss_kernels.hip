@@ -4,7 +4,7 @@
 // Reference behaviour followed (harness/primary/snapshot_sparse_seam_cuda_v1_0.cu):
 //   tile-local Komura labels        :194-289  -> tile_label()
 //   classify / number / emit        :291-475  -> tile_stage()
-//   seam merge (3 pairs per seam px):508-581  -> k_seam_merge
+//   seam merge (3 pairs per seam px):508-581  -> k_block_merge, k_group_merge, k_seam_merge
 //   compress + reduce               :583-619  -> k_resolve, k_final
 //   materialise seam records        :477-506  -> k_writeback
 // The reference writes a 1-byte record for every pixel and re-reads it for
