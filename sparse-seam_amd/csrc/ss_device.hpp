@@ -117,8 +117,9 @@ struct Ws {
     WsHeader* hdr;
     uint32_t* counts;    // [tiles] seam fragments per tile (+ invariant code)
     uint32_t* base;      // [tiles] dense id of the tile's first fragment
-    uint32_t* bsum;      // [nblocks + 1] scan block sums (last = total)
-    uint32_t* berr;      // [nblocks] OR of the tile invariant codes per scan block
+    uint64_t* bstate;    // [nblocks] look-back state of each scan block: flag (bits
+                         //   32-33: 0 none yet, 1 its own sum, 2 inclusive prefix) and
+                         //   value (bits 0-31); cleared by the tile stage
     uint8_t* sbits;      // [tiles * S] local G/S evidence per fragment slot (read by the
                          //   block merge only)
     uint8_t* flut;       // [tiles * S] per dense id, once resolved: final max / min mask

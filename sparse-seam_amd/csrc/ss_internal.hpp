@@ -19,7 +19,7 @@ struct Layout {
     int tile = 0;
     int slots = 0;
     uint32_t nblocks = 0;
-    size_t off_counts = 0, off_base = 0, off_bsum = 0, off_berr = 0, off_sbits = 0,
+    size_t off_counts = 0, off_base = 0, off_bstate = 0, off_sbits = 0,
            off_rec8 = 0, off_orows = 0, off_rec = 0, off_eq = 0, off_dparent = 0, off_dbits = 0,
            off_descs = 0, off_tile_img = 0, off_block_img = 0, off_seam_img = 0;
     size_t bytes = 0;
@@ -44,8 +44,7 @@ inline Layout make_layout(int64_t ntiles, int64_t nblk, int64_t nseam, int T, in
     };
     L.off_counts = take(nt * 4);
     L.off_base = take(nt * 4);
-    L.off_bsum = take(((size_t)L.nblocks + 1) * 4);
-    L.off_berr = take((size_t)L.nblocks * 4 + 4);
+    L.off_bstate = take((size_t)L.nblocks * 8);
     L.off_sbits = take(nt * S);
     L.off_rec8 = take(nt * (size_t)T * (size_t)T);
     L.off_orows = take(nt * 8);
@@ -67,8 +66,7 @@ inline Ws make_ws(void* base, const Layout& L, int threads) {
     w.hdr = reinterpret_cast<WsHeader*>(b);
     w.counts = reinterpret_cast<uint32_t*>(b + L.off_counts);
     w.base = reinterpret_cast<uint32_t*>(b + L.off_base);
-    w.bsum = reinterpret_cast<uint32_t*>(b + L.off_bsum);
-    w.berr = reinterpret_cast<uint32_t*>(b + L.off_berr);
+    w.bstate = reinterpret_cast<uint64_t*>(b + L.off_bstate);
     w.sbits = reinterpret_cast<uint8_t*>(b + L.off_sbits);
     w.flut = w.sbits;
     w.rec8 = reinterpret_cast<uint8_t*>(b + L.off_rec8);

@@ -18,6 +18,24 @@
 
 namespace ssx {
 
+// Set by the tile stage (one thread per tile), the first kernel of a run that
+// uses the workspace header: the header's per-run words (tile 0), and the
+// look-back state of each scan block (cleared by its first tile).
+__device__ __forceinline__ void run_prologue(int64_t gtile, const Ws& ws) {
+    if (gtile == 0) {
+        WsHeader* h = ws.hdr;
+        h->magic = WS_MAGIC;
+        h->total = 0;
+        h->error = 0;
+        h->dirty_count = 0;
+        h->retries = 0;
+        h->tiles = (uint64_t)ws.ntiles;
+        h->tile = ws.tile;
+        h->threads = ws.threads;
+    }
+    if (gtile % SCAN_CHUNK == 0) ws.bstate[gtile / SCAN_CHUNK] = 0;
+}
+
 // Plain form: one workgroup per tile, XCD-aware tile order.
 template <int T, int NT, bool MULTI>
 __global__ __launch_bounds__(NT) void k_tile_stage(Images ims, Ws ws) {
@@ -25,6 +43,7 @@ __global__ __launch_bounds__(NT) void k_tile_stage(Images ims, Ws ws) {
     const int64_t gtile = xcd_block(blockIdx.x, gridDim.x);
     if (gtile >= ws.ntiles) return;
     const ImgDev im = locate<MULTI>(ims, ws.tile_img, gtile);
+    if (threadIdx.x == 0) run_prologue(gtile, ws);
     tile_stage<T, NT>(sm, im, gtile, ws);
 }
 
@@ -38,6 +57,7 @@ __global__ __launch_bounds__(RT_THREADS * RT_NTW) void k_row_tile(Images ims, Ws
     const int64_t gtile = xcd_block(blockIdx.x, gridDim.x) * RT_NTW + sub;
     const bool live = gtile < ws.ntiles;
     const ImgDev im = live ? locate<MULTI>(ims, ws.tile_img, gtile) : ims.one;
+    if (live && threadIdx.x % RT_THREADS == 0) run_prologue(gtile, ws);
     row_tile_stage<MULTI>(sm[sub], im, gtile, ws, live, RT_NTW == 1 ? (int)threadIdx.x : (int)(threadIdx.x % RT_THREADS));
 }
 
@@ -77,76 +97,76 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave
     return r;
 }
 
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(Ws ws) {
-    __shared__ uint32_t s_wave[SCAN_BLOCK / 64 + 1];
-    __shared__ uint32_t s_err;
-    if (threadIdx.x == 0) s_err = 0;
-    const int64_t t0 = (int64_t)blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
-    uint32_t s = 0, e = 0;
-#pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i) {
-        if (t0 + i < ws.ntiles) {
-            const uint32_t c = ws.counts[t0 + i];
-            s += c & COUNT_MASK;
-            e |= c >> COUNT_ERR_SHIFT;
+// Exclusive prefix of the scan blocks' sums before block b (in launch order;
+// `own` is block b's sum), by a decoupled look-back, one wave: block b
+// publishes its own sum, then reads the states of the blocks before it, 64 at
+// a time, back to the nearest one that has published its inclusive prefix, and
+// publishes its own inclusive prefix.  A block waits only after publishing,
+// and blocks are dispatched in order, so a block it waits for runs or has run.
+// A wait that has not ended after LOOKBACK_SPINS reads sets invariant bit
+// 0x100 and returns (the run is then reported as not to be trusted) instead of
+// waiting on.
+constexpr uint32_t LOOKBACK_SPINS = 1u << 20;
+__device__ __forceinline__ uint32_t scan_lookback(const Ws& ws, uint32_t b, uint32_t own) {
+    constexpr uint64_t F_OWN = 1ull << 32, F_INC = 2ull << 32;
+    const int lane = threadIdx.x & 63;
+    if (lane == 0)
+        __hip_atomic_store(&ws.bstate[b], (b == 0 ? F_INC : F_OWN) | own, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t pre = 0, spins = 0;
+    int64_t j0 = (int64_t)b - 1;   // the window holds blocks j0 - lane
+    while (j0 >= 0) {
+        const int64_t j = j0 - lane;
+        const uint64_t st =
+            j >= 0 ? __hip_atomic_load(&ws.bstate[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : F_INC;
+        const uint32_t fl = (uint32_t)(st >> 32);
+        const uint64_t inc = __ballot(fl == 2u), none = __ballot(fl == 0u);
+        // lanes up to the nearest inclusive state (all, if the window has none)
+        const uint64_t upto = inc ? (inc & (~inc + 1ull)) * 2ull - 1ull : ~0ull;
+        if (none & upto) {
+            if (++spins > LOOKBACK_SPINS) {
+                if (lane == 0) atomicOr(&ws.hdr->error, 0x100u);
+                return pre;
+            }
+            __builtin_amdgcn_s_sleep(2);
+            continue;
         }
+        pre += wave_sum(((upto >> lane) & 1ull) ? (uint32_t)st : 0u);
+        if (inc) break;
+        j0 -= 64;
     }
-    uint32_t total;
-    block_excl_scan(s, s_wave, total);
-    if (e) atomicOr(&s_err, e);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        ws.bsum[blockIdx.x] = total;
-        ws.berr[blockIdx.x] = s_err;
-    }
+    if (lane == 0)
+        __hip_atomic_store(&ws.bstate[b], F_INC | (pre + own), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    return pre;
 }
 
-// Single workgroup: scan of the block sums; also folds the tile-stage
-// invariant word into the run's word and clears the per-run counters.
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_top(Ws ws) {
+// One pass: each workgroup scans SCAN_CHUNK tile counts, takes its offset by
+// the look-back, and writes its tiles' bases; the tile-stage invariant codes
+// are folded into the run's word and the last block writes the total.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan(Ws ws) {
     __shared__ uint32_t s_wave[SCAN_BLOCK / 64 + 1];
-    __shared__ uint32_t s_err;
-    if (threadIdx.x == 0) s_err = 0;
-    const uint32_t nb = ws.nblocks;
-    uint32_t carry = 0, err = 0;
-    for (uint32_t b0 = 0; b0 < nb; b0 += SCAN_BLOCK) {
-        const uint32_t i = b0 + threadIdx.x;
-        const uint32_t v = i < nb ? ws.bsum[i] : 0u;
-        if (i < nb) err |= ws.berr[i];
-        uint32_t total;
-        const uint32_t ex = block_excl_scan(v, s_wave, total);
-        if (i < nb) ws.bsum[i] = carry + ex;
-        carry += total;
-    }
-    if (err) atomicOr(&s_err, err);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        ws.bsum[nb] = carry;
-        WsHeader* h = ws.hdr;
-        h->magic = WS_MAGIC;
-        h->total = carry;
-        h->error = s_err;
-        h->dirty_count = 0;
-        h->retries = 0;
-        h->tiles = (uint64_t)ws.ntiles;
-        h->tile = ws.tile;
-        h->threads = ws.threads;
-    }
-}
-
-// Per scan block: the tile bases (dense id of each tile's first fragment).
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(Ws ws) {
-    __shared__ uint32_t s_wave[SCAN_BLOCK / 64 + 1];
+    __shared__ uint32_t s_pre;
     const int64_t t0 = (int64_t)blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
     uint32_t c[SCAN_ITEMS];
-    uint32_t s = 0;
+    uint32_t sum = 0, err = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
-        c[i] = (t0 + i < ws.ntiles) ? (ws.counts[t0 + i] & COUNT_MASK) : 0u;
-        s += c[i];
+        const uint32_t cw = (t0 + i < ws.ntiles) ? ws.counts[t0 + i] : 0u;
+        c[i] = cw & COUNT_MASK;
+        err |= cw >> COUNT_ERR_SHIFT;
+        sum += c[i];
     }
+    if (err) atomicOr(&ws.hdr->error, err);
     uint32_t total;
-    uint32_t run = block_excl_scan(s, s_wave, total) + ws.bsum[blockIdx.x];
+    uint32_t run = block_excl_scan(sum, s_wave, total);
+    if (threadIdx.x < 64) {
+        const uint32_t pre = scan_lookback(ws, blockIdx.x, total);
+        if (threadIdx.x == 0) {
+            s_pre = pre;
+            if (blockIdx.x == gridDim.x - 1) ws.hdr->total = pre + total;
+        }
+    }
+    __syncthreads();
+    run += s_pre;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
         if (t0 + i < ws.ntiles) ws.base[t0 + i] = run;
@@ -951,10 +971,7 @@ static hipError_t launch_all(const Images& ims, const Ws& ws, int T, int NT, int
     SSX_DISPATCH(T, NT, CALL_TILE)
 #undef CALL_TILE
     if (hook.tile_end) (void)hipEventRecord(hook.tile_end, s);
-    const uint32_t nb = ws.nblocks;
-    k_scan_reduce<<<nb, SCAN_BLOCK, 0, s>>>(ws);
-    k_scan_top<<<1, SCAN_BLOCK, 0, s>>>(ws);
-    k_scan_apply<<<nb, SCAN_BLOCK, 0, s>>>(ws);
+    k_scan<<<ws.nblocks, SCAN_BLOCK, 0, s>>>(ws);
     switch (T) {
     case 16: launch_merge<16, MULTI>(ims, ws, s); break;
     case 32: launch_merge<32, MULTI>(ims, ws, s); break;
