@@ -363,47 +363,46 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_block_merge(Images ims, Ws ws
     // edge may be partial) and whose first tile holds a fragment are visited:
     // their slots, in order, were listed by wave 0.
     const int NP = (int)s_nseam * T;
+    // The body is kept free of divergent branches up to the unions (LDS reads
+    // at clamped addresses, conditions folded into the pair mask): every
+    // divergent branch costs the wave scalar instructions for its exec mask.
 #pragma unroll 1
     for (int P0 = 0; P0 < NP; P0 += MERGE_THREADS) {
         const int P = P0 + threadIdx.x;
-        const int k = P % T;
-        const int sl = P < NP ? (int)slist[P / T] : 0;
+        const bool inp = P < NP;
+        const int Pc = inp ? P : 0;
+        const int k = Pc % T;
+        const int sl = (int)slist[Pc / T];
         const bool right = sl < NRS;
         const int sidx = right ? sl : sl - NRS;
         const int ix = right ? sidx % (MERGE_BT - 1) : sidx % MERGE_BT;
         const int iy = right ? sidx / (MERGE_BT - 1) : sidx / MERGE_BT;
         const int i = iy * MERGE_BT + ix;
-        uint32_t e = 0, ep = 0, en = 0;
-        if (P < NP) {
-            const uint8_t* eqs = seq[i] + (right ? 0 : T);
-            e = eqs[k];
-            ep = k > 0 ? eqs[k - 1] : 0u;
-            en = k < T - 1 ? eqs[k + 1] : 0u;
-        }
+        const uint8_t* eqs = seq[i] + (right ? 0 : T);
+        const uint32_t e = inp ? (uint32_t)eqs[k] : 0u;
+        const uint32_t ep = k > 0 ? (uint32_t)eqs[k > 0 ? k - 1 : 0] : 0u;
+        const uint32_t en = k < T - 1 ? (uint32_t)eqs[k < T - 1 ? k + 1 : k] : 0u;
         const bool s1 = (e & 2u) != 0;
         const bool u1 = s1 && !((e & 1u) && (ep & 2u));
         const bool u0 = (e & 1u) && (k == 0 || (!s1 && !(ep & 2u)));
         const bool u2 = (e & 4u) && (k == T - 1 || (!s1 && !(en & 2u)));
-        if (!e) continue;
         const uint32_t ci = lb[i + 1] - lb[i];
         const uint32_t ja = srec[i][right ? 3 * T + k : T + k];
-        if (ja >= ci) {
-            bad = true;
-            continue;
-        }
+        bad = bad || (e != 0u && ja >= ci);
         const uint32_t a = lb[i] + ja;
         // one union site for the (up to three) pairs of the position
-        uint32_t todo = (u0 ? 1u : 0u) | (u1 ? 2u : 0u) | (u2 ? 4u : 0u);
+        uint32_t todo = (e != 0u && ja < ci) ? ((u0 ? 1u : 0u) | (u1 ? 2u : 0u) | (u2 ? 4u : 0u)) : 0u;
         while (todo) {
             const int d = __builtin_ctz(todo);
             todo &= todo - 1u;
             const int nk = k + d - 1, dn = nk < 0 ? -1 : (nk >= T ? 1 : 0);
             const int niy = right ? iy + dn : iy + 1, nix = right ? ix + 1 : ix + dn;
-            if (niy < 0 || niy >= nty || nix < 0 || nix >= ntx) continue;   // another block's seam pair
-            const int nt = niy * MERGE_BT + nix;
+            const bool inb = niy >= 0 && niy < nty && nix >= 0 && nix < ntx;   // else another block's pair
+            const int nt = inb ? niy * MERGE_BT + nix : i;
             const uint32_t jb = srec[nt][(right ? 2 * T : 0) + nk - dn * T];
-            if (jb < lb[nt + 1] - lb[nt]) unite(a, i, lb[nt] + jb, nt);
-            else bad = true;
+            const uint32_t cn = lb[nt + 1] - lb[nt];
+            bad = bad || (inb && jb >= cn);
+            if (inb && jb < cn) unite(a, i, lb[nt] + jb, nt);
         }
     }
     __syncthreads();
