@@ -39,12 +39,13 @@ namespace ssx {
 constexpr int RT = 64;            // tile side
 constexpr int RT_THREADS = 128;   // two waves per tile
 constexpr int RT_NTW = 1;         // tiles per workgroup
-constexpr int RT_P = 80;          // staged row pitch: tile column c of staged row r at r * RT_P + 16 + c
+constexpr int RT_P = 72;          // staged row pitch: tile column c of staged row r at r * RT_P + 16 + c
+                                  // (a multiple of 8: the LDS footprint lets 12 workgroups share a CU)
 constexpr int RT_ROWS = 66;       // staged rows: tile rows -1 .. 64
 // Union queues (two waves) in the staged rows' LDS after the GS words
 // (the rows are no longer read when the unions run).
 constexpr int RT_UQ_OFF = 512;
-constexpr int RT_UQ = 592;        // node pairs per wave
+constexpr int RT_UQ = 528;        // node pairs per wave
 
 // Union-find nodes are indexed column-major, index 64 x + y for pixel (y, x),
 // so the entries the 64 lanes (rows) touch at one column are consecutive
@@ -73,7 +74,9 @@ struct RowTileSmem {
     };
 static_assert(RT_UQ_OFF + 2 * RT_UQ * 4 <= RT_ROWS * RT_P + 16, "union queues must fit in the staged rows");
     alignas(16) uint16_t lab[RT * RT];
-    uint8_t edge[4][RT];   // pixel values of tile row 0, row th - 1, column 0, column tw - 1
+    // edge pixels at 255 / at 0: bit i = pixel i of tile row 0, row th - 1,
+    // column 0, column tw - 1
+    uint64_t e255[4], e0[4];
     uint32_t count, open, err, zflags, nonuni, anygs, orow[2], qn[2];
 };
 
@@ -184,6 +187,17 @@ __device__ __forceinline__ uint32_t mask_bytes(uint32_t m, int d) {
 __device__ __forceinline__ uint32_t eq80(uint32_t a, uint32_t b) { return ~ne_top(a, b) & 0x80808080u; }
 __device__ __forceinline__ uint32_t bit(uint32_t m, int b) { return (m >> b) & 1u; }
 
+// 16 bytes of LDS at an 8-byte aligned address (the staged rows' pitch is a
+// multiple of 8, not of 16): two 8-byte accesses, which the compiler pairs.
+__device__ __forceinline__ uint4 lds_ld16(const uint8_t* p) {
+    const uint2 a = reinterpret_cast<const uint2*>(p)[0], b = reinterpret_cast<const uint2*>(p)[1];
+    return make_uint4(a.x, a.y, b.x, b.y);
+}
+__device__ __forceinline__ void lds_st16(uint8_t* p, const uint4& v) {
+    reinterpret_cast<uint2*>(p)[0] = make_uint2(v.x, v.y);
+    reinterpret_cast<uint2*>(p)[1] = make_uint2(v.z, v.w);
+}
+
 // Staged rows: tile rows -1 .. th (clamped to the image, so an out-of-image
 // halo pixel replicates the nearest image pixel; see tile_load), columns
 // 0 .. tw - 1 and the halo columns -1 and tw.
@@ -211,7 +225,7 @@ __device__ __forceinline__ void row_tile_load(RowTileSmem& sm, const ImgDev& im,
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) {
             const int it = t + i * RT_THREADS;
-            if (it < NV) *reinterpret_cast<uint4*>(&sm.img[(it >> 2) * RT_P + 16 + 16 * (it & 3)]) = v[i];
+            if (it < NV) lds_st16(&sm.img[(it >> 2) * RT_P + 16 + 16 * (it & 3)], v[i]);
         }
 #pragma unroll
         for (int i = 0; i < HALO; ++i) {
@@ -261,7 +275,7 @@ __device__ __forceinline__ void row_tile_load(RowTileSmem& sm, const ImgDev& im,
         if (r >= nrows || c >= tw) continue;
         uint8_t* dst = &sm.img[r * RT_P + 16 + c];
         if (c + 16 <= tw) {
-            *reinterpret_cast<uint4*>(dst) = v[i];
+            lds_st16(dst, v[i]);
         } else {
             // bytes past tw stay untouched: column tw is the halo column
             const uint32_t q[4] = {v[i].x, v[i].y, v[i].z, v[i].w};
@@ -287,7 +301,7 @@ __device__ __forceinline__ HalfRow read_half_row(const RowTileSmem& sm, int r, i
     const uint8_t* p = sm.img + (r + 1) * RT_P + 16 + 32 * h;
     HalfRow o;
     o.w[0] = *reinterpret_cast<const uint32_t*>(p - 4);
-    const uint4 q0 = *reinterpret_cast<const uint4*>(p), q1 = *reinterpret_cast<const uint4*>(p + 16);
+    const uint4 q0 = lds_ld16(p), q1 = lds_ld16(p + 16);
     o.w[1] = q0.x; o.w[2] = q0.y; o.w[3] = q0.z; o.w[4] = q0.w;
     o.w[5] = q1.x; o.w[6] = q1.y; o.w[7] = q1.z; o.w[8] = q1.w;
     o.w[9] = *reinterpret_cast<const uint32_t*>(p + 32);
@@ -322,16 +336,20 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
     }
     if (live) row_tile_load(sm, im, x0, y0, tw, th, tid);
     __syncthreads();
-    // The edge pixels' values, for the numbering of the seam zones after the
-    // staged rows have been reused (the values beyond tw / th are not read).
+    // Which edge pixels are at 255 / at 0, for the numbering of the seam
+    // zones after the staged rows have been reused (bits beyond tw / th are
+    // not read): wave 0 takes columns 0 and tw - 1, wave 1 rows 0 and th - 1.
     if (live) {
         const uint8_t* st = sm.img + 16;
-        if (h == 0) {
-            sm.edge[2][y] = st[(y + 1) * RT_P];
-            sm.edge[3][y] = st[(y + 1) * RT_P + tw - 1];
-        } else {
-            sm.edge[0][y] = st[RT_P + y];
-            sm.edge[1][y] = st[th * RT_P + y];
+        const uint32_t p = h == 0 ? st[(y + 1) * RT_P] : st[RT_P + y];
+        const uint32_t q = h == 0 ? st[(y + 1) * RT_P + tw - 1] : st[th * RT_P + y];
+        const uint64_t p255 = __ballot(p == 255u), p0 = __ballot(p == 0u);
+        const uint64_t q255 = __ballot(q == 255u), q0 = __ballot(q == 0u);
+        if (y == 0) {
+            sm.e255[h == 0 ? 2 : 0] = p255;
+            sm.e0[h == 0 ? 2 : 0] = p0;
+            sm.e255[h == 0 ? 3 : 1] = q255;
+            sm.e0[h == 0 ? 3 : 1] = q0;
         }
     }
 
@@ -693,8 +711,8 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
             // each candidate run's value: its pixel in the top / bottom row, or
             // the left / right edge pixel of its row (a middle row's seam run
             // holds column 0 or column tw - 1)
-            const uint8_t* const ev[2] = {&sm.edge[y < 32 ? 0 : 1][32 * h + bl],
-                                          &sm.edge[(cmid >> RL_COL) == 0 ? 2 : 3][y]};
+            const int ee[2] = {y < 32 ? 0 : 1, (cmid >> RL_COL) == 0 ? 2 : 3};
+            const int eb[2] = {32 * h + bl, y};
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 const uint32_t pc = c == 0 ? ctb : cmid;
@@ -719,8 +737,8 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                                                            __HIP_MEMORY_SCOPE_WORKGROUP) >> sh;
                 if (!(old & RL_ROOT)) {
                     uint32_t f = ((old >> RL_FSHIFT) & 7u) | (anygs && rl_gs(owngs, x) ? (F_G | F_S) : 0u) | F_SEAM;
-                    const uint32_t v = *ev[c];
-                    if (((f & F_G) || v == 255u) && ((f & F_S) || v == 0u)) f &= ~F_SEAM;
+                    const bool at255 = (sm.e255[ee[c]] >> eb[c]) & 1u, at0 = (sm.e0[ee[c]] >> eb[c]) & 1u;
+                    if (((f & F_G) || at255) && ((f & F_S) || at0)) f &= ~F_SEAM;
                     uint32_t id = RL_NOID;
                     const uint32_t j = atomicAdd(&sm.count, 1u);
                     if (j >= (uint32_t)S) {
@@ -821,8 +839,8 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
             for (int d = 0; d < 8; ++d)
                 cw[d] = REC_CLOSED * 0x01010101u | spread4((outMX >> (4 * d)) & 0xFu) |
                         spread4((outMN >> (4 * d)) & 0xFu) << 1;
-            *reinterpret_cast<uint4*>(row) = make_uint4(cw[0], cw[1], cw[2], cw[3]);
-            *reinterpret_cast<uint4*>(row + 16) = make_uint4(cw[4], cw[5], cw[6], cw[7]);
+            lds_st16(row, make_uint4(cw[0], cw[1], cw[2], cw[3]));
+            lds_st16(row + 16, make_uint4(cw[4], cw[5], cw[6], cw[7]));
             // the bytes of open runs after their first: filled from it
             const uint32_t OF = fill_fwd(opens, H) & ~opens;
             // A wave with long open runs writes each run's id at its first
@@ -849,7 +867,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                 }
             }
             if (dbl) {
-                const uint4 r0 = *reinterpret_cast<const uint4*>(row), r1 = *reinterpret_cast<const uint4*>(row + 16);
+                const uint4 r0 = lds_ld16(row), r1 = lds_ld16(row + 16);
                 uint32_t v[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
                 uint32_t f[8];
 #pragma unroll
@@ -871,8 +889,8 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
                         f[d] &= fs;
                     }
                 }
-                *reinterpret_cast<uint4*>(row) = make_uint4(v[0], v[1], v[2], v[3]);
-                *reinterpret_cast<uint4*>(row + 16) = make_uint4(v[4], v[5], v[6], v[7]);
+                lds_st16(row, make_uint4(v[0], v[1], v[2], v[3]));
+                lds_st16(row + 16, make_uint4(v[4], v[5], v[6], v[7]));
             }
         }
     } else if (live) {
@@ -909,7 +927,7 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         for (int i = 0; i < RT * RT / 16 / RT_THREADS; ++i) {
             const int c = tid + i * RT_THREADS;   // 16-byte chunk: row c / 4, part c % 4
             if ((orows >> (c >> 2)) & 1u)
-                dst[c] = *reinterpret_cast<const uint4*>(sm.plane + (c >> 2) * RT_P + 16 * (c & 3));
+                dst[c] = lds_ld16(sm.plane + (c >> 2) * RT_P + 16 * (c & 3));
         }
         if (tid == 0) ws.orows[gtile] = orows;
     }
