@@ -187,6 +187,22 @@ __device__ __forceinline__ uint32_t mask_bytes(uint32_t m, int d) {
 __device__ __forceinline__ uint32_t eq80(uint32_t a, uint32_t b) { return ~ne_top(a, b) & 0x80808080u; }
 __device__ __forceinline__ uint32_t bit(uint32_t m, int b) { return (m >> b) & 1u; }
 
+// Three-way packed maximum / minimum of 16-bit lanes that hold values 0 ..
+// 255.  Taken as half-precision bit patterns these are +0 and positive
+// subnormals, whose order is that of the integers; v_pk_maximum3_f16 /
+// v_pk_minimum3_f16 return one of their inputs (half-precision subnormals are
+// kept, not flushed, in the kernels' default floating-point mode), so the
+// result is the integer maximum / minimum in one instruction instead of two.
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_max3(uint32_t a, uint32_t b, uint32_t c) {
+    const f16x2 x = __builtin_bit_cast(f16x2, a), y = __builtin_bit_cast(f16x2, b), z = __builtin_bit_cast(f16x2, c);
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_maximum(__builtin_elementwise_maximum(x, y), z));
+}
+__device__ __forceinline__ uint32_t pk_min3(uint32_t a, uint32_t b, uint32_t c) {
+    const f16x2 x = __builtin_bit_cast(f16x2, a), y = __builtin_bit_cast(f16x2, b), z = __builtin_bit_cast(f16x2, c);
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_minimum(__builtin_elementwise_minimum(x, y), z));
+}
+
 // 16 bytes of LDS at an 8-byte aligned address (the staged rows' pitch is a
 // multiple of 8, not of 16): two 8-byte accesses, which the compiler pairs.
 __device__ __forceinline__ uint4 lds_ld16(const uint8_t* p) {
@@ -368,17 +384,17 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
         const uint32_t sel = 0x0C030C01u;
         const uint32_t ao = __builtin_amdgcn_perm(0u, a.w[0], sel), bo = __builtin_amdgcn_perm(0u, b.w[0], sel),
                        co = __builtin_amdgcn_perm(0u, c.w[0], sel);
-        pxo = pk_max(pk_max(ao, bo), co);
-        pno = pk_min(pk_min(ao, bo), co);
+        pxo = pk_max3(ao, bo, co);
+        pno = pk_min3(ao, bo, co);
     }
     auto vert = [&](int i, uint32_t& xe, uint32_t& xo, uint32_t& ne, uint32_t& no) {
         const uint32_t ae = __builtin_amdgcn_perm(0u, a.w[i], 0x0C020C00u), ao = __builtin_amdgcn_perm(0u, a.w[i], 0x0C030C01u);
         const uint32_t be = __builtin_amdgcn_perm(0u, b.w[i], 0x0C020C00u), bo = __builtin_amdgcn_perm(0u, b.w[i], 0x0C030C01u);
         const uint32_t ce = __builtin_amdgcn_perm(0u, c.w[i], 0x0C020C00u), co = __builtin_amdgcn_perm(0u, c.w[i], 0x0C030C01u);
-        xe = pk_max(pk_max(ae, be), ce);
-        xo = pk_max(pk_max(ao, bo), co);
-        ne = pk_min(pk_min(ae, be), ce);
-        no = pk_min(pk_min(ao, bo), co);
+        xe = pk_max3(ae, be, ce);
+        xo = pk_max3(ao, bo, co);
+        ne = pk_min3(ae, be, ce);
+        no = pk_min3(ao, bo, co);
     };
     uint32_t cxe, cxo, cne, cno;   // current dword: even / odd max / min
     vert(1, cxe, cxo, cne, cno);
@@ -393,18 +409,17 @@ __device__ __forceinline__ void row_tile_stage(RowTileSmem& sm, const ImgDev& im
             const uint32_t sel = 0x0C020C00u;
             const uint32_t ae = __builtin_amdgcn_perm(0u, a.w[9], sel), be = __builtin_amdgcn_perm(0u, b.w[9], sel),
                            ce = __builtin_amdgcn_perm(0u, c.w[9], sel);
-            nxe = pk_max(pk_max(ae, be), ce);
-            nne = pk_min(pk_min(ae, be), ce);
+            nxe = pk_max3(ae, be, ce);
+            nne = pk_min3(ae, be, ce);
         }
         // lanes (pixel 0, pixel 2) take (column -1, column 1), lanes (pixel 1,
         // pixel 3) take (column 2, column 4); back in byte form a strictly
         // greater neighbour exists iff the maximum differs from the pixel, a
         // strictly smaller one iff the minimum does
-        const uint32_t M = pk_max(cxe, cxo), N = pk_min(cne, cno);
-        const uint32_t mx3 = pk_max(M, __builtin_amdgcn_alignbit(cxo, pxo, 16)) |
-                             pk_max(M, __builtin_amdgcn_alignbit(nxe, cxe, 16)) << 8;
-        const uint32_t mn3 = pk_min(N, __builtin_amdgcn_alignbit(cno, pno, 16)) |
-                             pk_min(N, __builtin_amdgcn_alignbit(nne, cne, 16)) << 8;
+        const uint32_t mx3 = pk_max3(cxe, cxo, __builtin_amdgcn_alignbit(cxo, pxo, 16)) |
+                             pk_max3(cxe, cxo, __builtin_amdgcn_alignbit(nxe, cxe, 16)) << 8;
+        const uint32_t mn3 = pk_min3(cne, cno, __builtin_amdgcn_alignbit(cno, pno, 16)) |
+                             pk_min3(cne, cno, __builtin_amdgcn_alignbit(nne, cne, 16)) << 8;
         const uint32_t V = b.w[d + 1];
         const uint32_t eR = eq80(V, __builtin_amdgcn_alignbyte(b.w[d + 2], V, 1));
         const uint32_t eD = eq80(V, c.w[d + 1]);
