@@ -51,7 +51,7 @@ constexpr uint32_t LMASK = 0x3FFFu;     // label bits (T*T <= 16384: the profile
 constexpr int FLAG_SHIFT = 16;          // zone evidence OR'ed at the root
 constexpr int FRAG_SHIFT = 20;          // fragment id + 1 stored at seam roots
 constexpr uint32_t DB_BROOT = 16u;      // dense bits: root of a block-local component
-constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_BLOCK = 1024;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_CHUNK = SCAN_BLOCK * SCAN_ITEMS;  // tiles per scan block
 constexpr int NUM_XCD = 8;
