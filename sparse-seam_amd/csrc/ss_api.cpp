@@ -187,7 +187,7 @@ static ss_status batch_prepare(const ss_image_desc* images, int n, const ss_conf
     if (!images || n <= 0) return SS_ERR_ARG;
     if (!config_valid(c.tile, c.threads)) return SS_ERR_ARG;
     table.resize((size_t)n);
-    int64_t tiles = 0, blocks = 0, seams = 0;
+    int64_t tiles = 0, blocks = 0, seams = 0, groups = 0;
     for (int i = 0; i < n; ++i) {
         const ss_image_desc& d = images[i];
         if (d.width <= 0 || d.height <= 0 || d.pitch < (uint64_t)d.width || d.out_pitch < (uint64_t)d.width)
@@ -195,12 +195,14 @@ static ss_status batch_prepare(const ss_image_desc* images, int n, const ss_conf
         ImgDev& t = table[(size_t)i];
         t = make_imgdev(d.img, d.max, d.min, (int64_t)d.pitch, (int64_t)d.out_pitch, d.width, d.height, c.tile,
                         tiles, blocks, seams);
+        t.group_begin = groups;
+        groups += groups_of(t);
         tiles += (int64_t)t.tiles_x * t.tiles_y;
         blocks += blocks_of(t);
         seams += seams_of(t);
     }
     if (tiles * (4 * c.tile - 4) > (int64_t)0xFFFFFFF0u) return SS_ERR_CAPACITY;
-    L = make_layout(tiles, blocks, seams, c.tile, n);
+    L = make_layout(tiles, blocks, seams, c.tile, n, groups);
     return SS_OK;
 }
 

@@ -75,6 +75,9 @@ struct ImgDev {
     // row but the last (seam_rows x tiles_x)
     int64_t seam_begin;
     int32_t seam_cols, seam_rows;
+    // global index of its first group of 2 x 2 merge blocks (group merge;
+    // groups row-major, ceil(blocks_x / 2) per row)
+    int64_t group_begin;
 };
 
 struct Images {
@@ -138,9 +141,11 @@ struct Ws {
     uint32_t* tile_img;  // [tiles] image index of each tile (batches only)
     uint32_t* block_img; // [nblk] image index of each merge block (batches only)
     uint32_t* seam_img;  // [nseam] image index of each seam-merge unit (batches only)
+    uint32_t* group_img; // [ngroup] image index of each merge-block group (batches only)
     int64_t ntiles;
     int64_t nblk;        // merge blocks
     int64_t nseam;       // seam-merge units (one tile edge each)
+    int64_t ngroup;      // groups of 2 x 2 merge blocks
     uint32_t nblocks;    // scan blocks = ceil(ntiles / SCAN_CHUNK)
     int tile;            // T
     int threads;         // threads per tile

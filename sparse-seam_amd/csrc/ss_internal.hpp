@@ -16,22 +16,24 @@ struct Layout {
     int64_t ntiles = 0;
     int64_t nblk = 0;
     int64_t nseam = 0;
+    int64_t ngroup = 0;
     int tile = 0;
     int slots = 0;
     uint32_t nblocks = 0;
     size_t off_counts = 0, off_base = 0, off_bstate = 0, off_sbits = 0,
            off_rec8 = 0, off_orows = 0, off_rec = 0, off_eq = 0, off_dparent = 0, off_dbits = 0,
-           off_descs = 0, off_tile_img = 0, off_block_img = 0, off_seam_img = 0;
+           off_descs = 0, off_tile_img = 0, off_block_img = 0, off_seam_img = 0, off_group_img = 0;
     size_t bytes = 0;
 };
 
 inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
-inline Layout make_layout(int64_t ntiles, int64_t nblk, int64_t nseam, int T, int ndesc) {
+inline Layout make_layout(int64_t ntiles, int64_t nblk, int64_t nseam, int T, int ndesc, int64_t ngroup = 0) {
     Layout L;
     L.ntiles = ntiles;
     L.nblk = nblk;
     L.nseam = nseam;
+    L.ngroup = ngroup;
     L.tile = T;
     L.slots = 4 * T - 4;
     L.nblocks = (uint32_t)((ntiles + SCAN_CHUNK - 1) / SCAN_CHUNK);
@@ -56,6 +58,7 @@ inline Layout make_layout(int64_t ntiles, int64_t nblk, int64_t nseam, int T, in
     L.off_tile_img = take(ndesc > 1 ? nt * 4 : 0);
     L.off_block_img = take(ndesc > 1 ? (size_t)nblk * 4 : 0);
     L.off_seam_img = take(ndesc > 1 ? (size_t)nseam * 4 : 0);
+    L.off_group_img = take(ndesc > 1 ? (size_t)ngroup * 4 : 0);
     L.bytes = o;
     return L;
 }
@@ -78,9 +81,11 @@ inline Ws make_ws(void* base, const Layout& L, int threads) {
     w.tile_img = reinterpret_cast<uint32_t*>(b + L.off_tile_img);
     w.block_img = reinterpret_cast<uint32_t*>(b + L.off_block_img);
     w.seam_img = reinterpret_cast<uint32_t*>(b + L.off_seam_img);
+    w.group_img = reinterpret_cast<uint32_t*>(b + L.off_group_img);
     w.ntiles = L.ntiles;
     w.nblk = L.nblk;
     w.nseam = L.nseam;
+    w.ngroup = L.ngroup;
     w.nblocks = L.nblocks;
     w.tile = L.tile;
     w.threads = threads;
@@ -122,6 +127,7 @@ inline ImgDev make_imgdev(const uint8_t* img, uint8_t* mx, uint8_t* mn, int64_t 
     return d;
 }
 inline int64_t blocks_of(const ImgDev& d) { return (int64_t)d.blocks_x * d.blocks_y; }
+inline int64_t groups_of(const ImgDev& d) { return (int64_t)((d.blocks_x + 1) / 2) * ((d.blocks_y + 1) / 2); }
 inline int64_t seams_of(const ImgDev& d) {
     return (int64_t)d.seam_cols * d.tiles_y + (int64_t)d.seam_rows * d.tiles_x;
 }

@@ -71,6 +71,8 @@ __global__ __launch_bounds__(256) void k_tile_map(Images ims, Ws ws) {
     for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) ws.block_img[im.block_begin + b] = blockIdx.x;
     const int64_t ns = (int64_t)im.seam_cols * im.tiles_y + (int64_t)im.seam_rows * im.tiles_x;
     for (int64_t u = threadIdx.x; u < ns; u += blockDim.x) ws.seam_img[im.seam_begin + u] = blockIdx.x;
+    const int64_t ng = (int64_t)((im.blocks_x + 1) / 2) * ((im.blocks_y + 1) / 2);
+    for (int64_t g = threadIdx.x; g < ng; g += blockDim.x) ws.group_img[im.group_begin + g] = blockIdx.x;
 }
 
 // ---- compaction: exclusive scan of the per-tile counts --------------------
@@ -526,8 +528,7 @@ __device__ __forceinline__ void edge_pair_roots(const Ws& ws, const ImgDev& im, 
 // it are merged here, in LDS, by one workgroup that alone touches the group's
 // block roots, so its links are plain stores; the seam merge then takes only
 // the seams between groups (and the diagonal contacts that leave a group at a
-// tile corner).  One workgroup per group works on it (in a batch, that of the
-// group's top-left block; the others exit): its edges are the right edges of the tiles in the last
+// tile corner).  One workgroup per group works on it: its edges are the right edges of the tiles in the last
 // tile column of its block column (vertical seam, GROUP_T tiles) and the
 // bottom edges of the tiles in the last tile row of its block row (horizontal
 // seam), T positions each.  The group's distinct block roots get LDS slots in
@@ -550,20 +551,10 @@ __global__ __launch_bounds__(GROUP_THREADS) void k_group_merge(Images ims, Ws ws
     __shared__ uint32_t gmin[GSET];   // smallest id of a component, at its root slot
     __shared__ uint32_t used[GSET];   // the slots in use
     __shared__ uint32_t s_over, s_used;
-    // one workgroup per group (single image), or per merge block of a batch
-    // (the others exit)
-    const ImgDev im = MULTI ? ims.table[ws.block_img[blockIdx.x]] : ims.one;
-    int bx, by;
-    if (MULTI) {
-        const uint32_t bl = (uint32_t)((int64_t)blockIdx.x - im.block_begin);
-        bx = (int)(bl % (uint32_t)im.blocks_x);
-        by = (int)(bl / (uint32_t)im.blocks_x);
-        if ((bx | by) & 1) return;
-    } else {
-        const uint32_t gx = (uint32_t)(im.blocks_x + 1) / 2;
-        bx = 2 * (int)(blockIdx.x % gx);
-        by = 2 * (int)(blockIdx.x / gx);
-    }
+    // one workgroup per group
+    const ImgDev im = MULTI ? ims.table[ws.group_img[blockIdx.x]] : ims.one;
+    const uint32_t gl = (uint32_t)((int64_t)blockIdx.x - im.group_begin), gx = (uint32_t)(im.blocks_x + 1) / 2;
+    const int bx = 2 * (int)(gl % gx), by = 2 * (int)(gl / gx);
     const bool hasV = bx + 1 < im.blocks_x, hasH = by + 1 < im.blocks_y;
     if (!hasV && !hasH) return;
     for (int i = threadIdx.x; i < GSET; i += GROUP_THREADS) {
@@ -946,7 +937,7 @@ static void launch_writeback(const Images& ims, const Ws& ws, int grid_cap, hipS
 template <int T, bool MULTI>
 static void launch_merge(const Images& ims, const Ws& ws, hipStream_t s) {
     k_block_merge<T, MULTI><<<(unsigned)ws.nblk, MERGE_THREADS, 0, s>>>(ims, ws);
-    const int64_t groups = MULTI ? ws.nblk : (int64_t)((ims.one.blocks_x + 1) / 2) * ((ims.one.blocks_y + 1) / 2);
+    const int64_t groups = MULTI ? ws.ngroup : (int64_t)((ims.one.blocks_x + 1) / 2) * ((ims.one.blocks_y + 1) / 2);
     k_group_merge<T, MULTI><<<(unsigned)groups, GROUP_THREADS, 0, s>>>(ims, ws);
     const int64_t merge_threads = ws.nseam * T;
     if (merge_threads > 0) {
