@@ -657,8 +657,8 @@ __global__ __launch_bounds__(GROUP_THREADS) void k_group_merge(Images ims, Ws ws
 // down-left contact of the bottom edge of the tile above-right of it (resp.
 // the down-right contact of both edges of one tile).  Pairs inside one block
 // were united by k_block_merge, pairs inside a 2 x 2 group of blocks by
-// k_group_merge: of a seam inside a group only the diagonal pairs at a tile
-// corner that reach another group are left.
+// k_group_merge; the edges on seams inside a group have nothing left (see
+// below).
 //
 // A plateau crossing a block seam yields the same pair of block components at
 // many pixels of the seam, and every union ends in a device-scope CAS (or
@@ -675,8 +675,6 @@ template <int T, bool MULTI>
 __global__ __launch_bounds__(SEAM_THREADS) void k_seam_merge(Images ims, Ws ws) {
     constexpr unsigned long long EMPTY = ~0ull;
     __shared__ unsigned long long set[SEAM_SET];
-    for (int i = threadIdx.x; i < SEAM_SET; i += SEAM_THREADS) set[i] = EMPTY;
-    __syncthreads();
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t unit = g / T;
     const int k = (int)(g % T);
@@ -685,8 +683,8 @@ __global__ __launch_bounds__(SEAM_THREADS) void k_seam_merge(Images ims, Ws ws) 
     EdgeWords w;
     ImgDev im{};
     int tx = 0, ty = 0;
-    bool right = false, inner = false;
-    int along = 0;
+    bool right = false, inner = true;
+    int64_t gtile = 0;
     if (unit < ws.nseam) {
         im = MULTI ? ims.table[ws.seam_img[unit]] : ims.one;
         const uint32_t u = (uint32_t)(unit - im.seam_begin);
@@ -694,19 +692,22 @@ __global__ __launch_bounds__(SEAM_THREADS) void k_seam_merge(Images ims, Ws ws) 
                                (uint32_t)im.tiles_y, tx, ty);
         // a seam inside a group: between block columns (rows) 2 i and 2 i + 1
         inner = (((right ? tx : ty) / MERGE_BT) & 1) == 0;
-        along = right ? ty : tx;
-        const bool need = !inner || (k == 0 && along % GROUP_T == 0) || (k == T - 1 && (along + 1) % GROUP_T == 0);
         // by construction the tile right of (below) the edge exists
-        const int64_t gtile = im.tile_begin + (int64_t)ty * im.tiles_x + tx;
-        if (need) w = load_edge<T>(ws, gtile, right ? gtile + 1 : gtile + im.tiles_x, right, k);
+        gtile = im.tile_begin + (int64_t)ty * im.tiles_x + tx;
     }
+    // An edge on a seam inside a group has nothing left: k_group_merge united
+    // the pairs inside the group, and each diagonal contact at one of its tile
+    // corners that reaches another group is also the corner contact of an
+    // edge on a seam between groups (the right edge of the tile below-left of
+    // it, the bottom edge of the tile above-right, or the other edge of the
+    // same tile), which this kernel takes.  A workgroup of such edges only
+    // (one block seam at T = 64) leaves at once.
+    if (__syncthreads_and(inner)) return;
+    for (int i = threadIdx.x; i < SEAM_SET; i += SEAM_THREADS) set[i] = EMPTY;
+    __syncthreads();
+    if (!inner) w = load_edge<T>(ws, gtile, right ? gtile + 1 : gtile + im.tiles_x, right, k);
     bool use[3];
     edge_pairs_used<T>(w.e, k, use);
-    if (inner) {
-        use[1] = false;
-        use[0] = use[0] && k == 0 && along % GROUP_T == 0;
-        use[2] = use[2] && k == T - 1 && (along + 1) % GROUP_T == 0;
-    }
     uint32_t ra, rb[3];
     edge_pair_roots<T>(ws, im, w, tx, ty, right, k, use, ra, rb, bad);
     bool keep[3];
